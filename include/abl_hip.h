@@ -1,0 +1,181 @@
+/*
+ * abl_hip.h -- C ABI of the MI355X (gfx950) OpenABL runtime.
+ *
+ * This is the run-time boundary between code generated for `-b hip` and the
+ * device runtime.  It replaces, for that backend, the reference's run-time
+ * interface asset/c/libabl.h:1-217 (generator, RTTI, save) and the simulate
+ * loop the C backend prints inline (src/backend/CPrinter.cpp:148-233).
+ * Every entry point is plain C: opaque handles, pointers and sizes, an int
+ * status (0 = ABL_OK) and a thread-local message from abl_hip_last_error().
+ *
+ * Agent data crosses the boundary in exactly the C backend's struct layout
+ * (src/backend/CPrinter.cpp:250-265: members in declaration order, natural
+ * alignment, float members as double or, with use_float, float), so host
+ * initialisation code and save() stay byte-identical to the C backend.
+ * Inside the runtime agents live in HBM as struct-of-arrays, double-buffered,
+ * sorted by grid cell.
+ *
+ * Thread safety: distinct handles may be used from distinct threads
+ * concurrently; one handle must not be used from two threads at once.
+ */
+#ifndef ABL_HIP_H
+#define ABL_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  ABL_OK = 0,
+  ABL_ERR_ARG = 1,          /* invalid argument (unknown name, bad index, size) */
+  ABL_ERR_STATE = 2,        /* call not valid in the handle's current state */
+  ABL_ERR_DEVICE = 3,       /* HIP runtime error (message names the call) */
+  ABL_ERR_MEMORY = 4,       /* device or host allocation failed / over budget */
+  ABL_ERR_UNSUPPORTED = 5,  /* feature not supported (the reference's exit 2,
+                               src/main.cpp:270-274) */
+};
+
+/* Human-readable message of the last failing call on this thread. */
+const char *abl_hip_last_error(void);
+
+/* ---- run-time type information ----------------------------------------
+ * Same values and layout as the reference's type_id / type_info /
+ * agent_info (asset/c/libabl.h:188-209), so generated RTTI tables can be
+ * passed unchanged. */
+typedef enum {
+  ABL_TYPE_END = 0,
+  ABL_TYPE_BOOL = 1,
+  ABL_TYPE_INT = 2,
+  ABL_TYPE_FLOAT = 3,
+  ABL_TYPE_STRING = 4,
+  ABL_TYPE_FLOAT2 = 5,
+  ABL_TYPE_FLOAT3 = 6,
+} abl_type_id;
+
+typedef struct {
+  abl_type_id type;
+  unsigned offset;
+  const char *name;
+  bool is_pos;
+} abl_type_info;
+
+typedef struct {
+  const abl_type_info *info;
+  unsigned offset;
+  const char *name;
+} abl_agent_info;
+
+/* The reference's dyn_array (asset/c/libabl.h:11-15). */
+typedef struct {
+  void *values;
+  size_t len;
+  size_t cap;
+} abl_dyn_array;
+
+/* ---- host runtime (replaces asset/c/libabl.c for -b hip) ---------------
+ * xorshift128+ from the fixed state {0xdeadbeef, 0xbeefdead}
+ * (asset/c/libabl.c:8-39).  The generator is per process, like the
+ * reference's; abl_rng_seed() replaces the state (the `hip.seed` config). */
+double abl_random_float(double lo, double hi);      /* FP64 build semantics */
+float abl_random_floatf(float lo, float hi);         /* use_float semantics */
+int abl_random_int(int lo, int hi);
+void abl_rng_seed(uint64_t s0, uint64_t s1);
+void abl_rng_state(uint64_t out[2]);
+
+/* save(agents, info, path, SAVE_JSON) (asset/c/libabl.c:46-124, 215-231):
+ * `agents` points at the generated `struct agent_struct` whose dyn_arrays
+ * the agent_info offsets address; float_bytes is 8, or 4 under use_float.
+ * Returns ABL_ERR_ARG if the file cannot be opened (the reference prints to
+ * stderr and returns). */
+int abl_save_json(const void *agents, const abl_agent_info *info, const char *path,
+                  int float_bytes);
+
+/* ---- simulations ------------------------------------------------------- */
+typedef struct abl_sim abl_sim;
+
+typedef enum {
+  ABL_PREC_F64 = 0,   /* the C backend's default (double) */
+  ABL_PREC_F32 = 1,   /* -C use_float=true (asset/c/libabl.h:59-63) */
+  ABL_PREC_F32C = 2,  /* FP32 neighbour arithmetic with compensated sums,
+                         FP64 positions and integration */
+} abl_precision;
+
+/* Step programs compiled into this library (the device code `-b hip` emits
+ * per model).  Names: "circle", "circle3d", "boids2d", "game_of_life". */
+int abl_hip_program_count(void);
+const char *abl_hip_program_name(int index);
+
+/* Create a simulation of `program` at `precision` on the current device.
+ * Parameters start at the model's defaults. */
+int abl_hip_create(const char *program, int precision, abl_sim **out);
+int abl_hip_destroy(abl_sim *sim);
+
+/* `-P name=value`: set a model parameter (also derived constants, which the
+ * caller folds as the reference compiler does).  Only before the first
+ * simulate call after an upload. */
+int abl_hip_set_param(abl_sim *sim, const char *name, double value);
+int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
+
+/* `-C key=value` runtime options: "hip.block" (threads per block),
+ * "hip.cell_scale" (cells per granularity, 1 or 2), "hip.order"
+ * ("cell" or "index": visit neighbours in cell order or in the reference's
+ * index order). */
+int abl_hip_set_config(abl_sim *sim, const char *key, const char *value);
+
+/* Agent types of the program, in declaration order. */
+int abl_hip_agent_type_count(abl_sim *sim);
+const char *abl_hip_agent_type_name(abl_sim *sim, int agent_type);
+/* Generated-struct size of an agent type at the sim's precision. */
+size_t abl_hip_agent_size(abl_sim *sim, int agent_type);
+
+/* Replace the agents of one type with `count` records in the generated struct
+ * layout (the host's `agents.agents_T.values`).  Record i becomes agent id i. */
+int abl_hip_upload(abl_sim *sim, int agent_type, const void *aos, size_t count);
+/* Copy the agents of one type back in id order (the reference's array
+ * order).  `capacity` is in records; `count` receives the number alive. */
+int abl_hip_download(abl_sim *sim, int agent_type, void *aos, size_t capacity,
+                     size_t *count);
+int abl_hip_count(abl_sim *sim, int agent_type, size_t *count);
+
+/* Run the program's main() statements before simulate(...) on the host --
+ * the agent-creation loops, drawing from the process generator
+ * (abl_random_float) in the C backend's evaluation order -- and upload the
+ * result.  Replaces the init half of the generated main()
+ * (src/backend/CPrinter.cpp:69-73, 84-90, 267-272). */
+int abl_hip_init_agents(abl_sim *sim);
+
+/* Upload / download every type through the generated agent_struct and
+ * agent_info table (the same arguments save() takes). */
+int abl_hip_upload_all(abl_sim *sim, const void *agents, const abl_agent_info *info);
+int abl_hip_download_all(abl_sim *sim, void *agents, const abl_agent_info *info);
+
+/* Run `timesteps` timesteps of every step function in program order
+ * (src/backend/CPrinter.cpp:189-233).  Asynchronous on the sim's stream. */
+int abl_hip_simulate(abl_sim *sim, int timesteps);
+/* Wait for the sim's stream. */
+int abl_hip_synchronize(abl_sim *sim);
+
+/* Parity probe: the (centre id, neighbour id) pairs that step function
+ * `step`'s near loop accepts on the current state, sorted by centre then
+ * neighbour id.  Pass pairs == NULL to query the count. */
+int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t capacity,
+                       size_t *count);
+
+/* Device memory the sim holds, in bytes. */
+int abl_hip_footprint(abl_sim *sim, size_t *bytes);
+
+/* Timing of the last simulate call's kernels, in milliseconds, measured with
+ * HIP events on the sim's stream: [0] whole call, [1] neighbour-step kernels,
+ * [2] grid build (bin, scan, sort, gather).  Requires "hip.timing"="1". */
+int abl_hip_last_timing(abl_sim *sim, double out_ms[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

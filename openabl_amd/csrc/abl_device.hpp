@@ -1,0 +1,200 @@
+// abl_device.hpp -- device-side building blocks shared by the runtime's
+// grid pipeline and the step programs (gfx950, wave64).
+//
+// Exactness rules the step programs rely on (all code here is compiled with
+// -ffp-contract=off, so a*b+c is never fused):
+//   * the near test of the reference C backend is
+//       (double) sqrtf((float)(dx*dx + dy*dy [+ dz*dz])) <= r
+//     (src/backend/CPrinter.cpp:168-170, asset/c/libabl.h:156-168).  sqrtf is
+//     correctly rounded and monotone, so that test equals
+//       (float)(dx*dx + dy*dy [+ dz*dz]) <= T,   T = max{f : sqrtf(f) <= r},
+//     with T found on the host; the device never takes a square root for it.
+//   * where a step needs the distance value itself, sqrt_rn_f32() returns the
+//     correctly rounded single-precision root: the double root of a float is
+//     correctly rounded, and rounding it again to float is exact because
+//     53 >= 2*24 + 2 (double rounding is innocuous for sqrt).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abl {
+
+constexpr int kMaxCols = 16;
+constexpr int kWave = 64;
+
+// Correctly rounded sqrtf.
+__device__ __forceinline__ float sqrt_rn_f32(float f) {
+  return (float) __builtin_sqrt((double) f);
+}
+
+// Uniform grid of one agent type: cell edge = granularity / cell_scale,
+// widened by 2^-20 so float rounding in the near test can never reach past
+// the stencil; `reach` cells are searched on each side.
+struct GridView {
+  double lo[3];
+  double inv_cell;   // 1 / cell edge
+  int n[3];          // cells per axis (n[2] == 1 in 2-D)
+  int reach;
+  int dim;
+  const uint32_t *start;  // ncells + 1 offsets into the cell-sorted order
+};
+
+__device__ __forceinline__ int cell_coord(double p, double lo, double inv, int n) {
+  double q = floor((p - lo) * inv);
+  // !(q >= 0) also maps NaN to cell 0.
+  int c = !(q >= 0.0) ? 0 : (q > (double) (n - 1) ? n - 1 : (int) q);
+  return c;
+}
+
+// Square of the reference's vector difference nx.pos - in.pos in abl_float
+// arithmetic, rounded to float as the sqrtf argument is.
+template <typename F>
+__device__ __forceinline__ float dist_sq_f32(F ax, F ay, F bx, F by) {
+  F dx = ax - bx;
+  F dy = ay - by;
+  F s = dx * dx + dy * dy;
+  return (float) s;
+}
+template <typename F>
+__device__ __forceinline__ float dist_sq_f32(F ax, F ay, F az, F bx, F by, F bz) {
+  F dx = ax - bx;
+  F dy = ay - by;
+  F dz = az - bz;
+  F s = dx * dx + dy * dy + dz * dz;
+  return (float) s;
+}
+
+// Inclusive-sum scan across one wave64.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t u = __shfl_up(v, d, kWave);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan (blockDim.x a multiple of 64, at most 1024).
+// Returns the exclusive prefix; *total receives the block sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds_waves,
+                                                    uint32_t *total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  uint32_t incl = wave_incl_scan(v);
+  if (lane == kWave - 1) lds_waves[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < nwaves ? lds_waves[lane] : 0u;
+    uint32_t wi = wave_incl_scan(w);
+    if (lane < nwaves) lds_waves[lane] = wi - w;  // exclusive wave offsets
+    if (lane == nwaves - 1) lds_waves[kWave] = wi;
+  }
+  __syncthreads();
+  uint32_t res = incl - v + lds_waves[wave];
+  *total = lds_waves[kWave];
+  __syncthreads();
+  return res;
+}
+
+}  // namespace abl
+
+namespace abl {
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X (each with
+// its own L2).  Remap so that every XCD works on one contiguous range of the
+// cell-sorted agents: neighbours of a block's agents then sit in the same
+// XCD's L2 instead of being fetched by all eight.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned x = b & 7u, k = b >> 3;
+  const unsigned q = nb >> 3, rem = nb & 7u;
+  return x * q + (x < rem ? x : rem) + k;
+}
+
+// Visit the candidate agents of a stencil in cell-sorted order: for each row
+// of cells (fixed y, z) the x-range of cells is one contiguous slot range.
+template <typename Visit>
+__device__ __forceinline__ void for_each_candidate(const GridView &g, int cx, int cy, int cz,
+                                                   int reach, Visit &&visit) {
+  const int x0 = cx - reach < 0 ? 0 : cx - reach;
+  const int x1 = cx + reach > g.n[0] - 1 ? g.n[0] - 1 : cx + reach;
+  const int y0 = cy - reach < 0 ? 0 : cy - reach;
+  const int y1 = cy + reach > g.n[1] - 1 ? g.n[1] - 1 : cy + reach;
+  int z0 = 0, z1 = 0;
+  if (g.dim == 3) {
+    z0 = cz - reach < 0 ? 0 : cz - reach;
+    z1 = cz + reach > g.n[2] - 1 ? g.n[2] - 1 : cz + reach;
+  }
+  for (int z = z0; z <= z1; z++) {
+    for (int y = y0; y <= y1; y++) {
+      const uint32_t row = ((uint32_t) z * (uint32_t) g.n[1] + (uint32_t) y) * (uint32_t) g.n[0];
+      const uint32_t a = g.start[row + x0];
+      const uint32_t b = g.start[row + x1 + 1];
+      for (uint32_t j = a; j < b; j++) visit(j);
+    }
+  }
+}
+
+// Same candidates in ascending id order (the reference's visiting order):
+// ids ascend inside every cell, so a merge over the stencil's cells yields
+// the global id order.
+template <typename Visit>
+__device__ __forceinline__ void for_each_candidate_by_id(const GridView &g, int cx, int cy, int cz,
+                                                         int reach, const int32_t *ids,
+                                                         Visit &&visit) {
+  const int x0 = cx - reach < 0 ? 0 : cx - reach;
+  const int x1 = cx + reach > g.n[0] - 1 ? g.n[0] - 1 : cx + reach;
+  const int y0 = cy - reach < 0 ? 0 : cy - reach;
+  const int y1 = cy + reach > g.n[1] - 1 ? g.n[1] - 1 : cy + reach;
+  int z0 = 0, z1 = 0;
+  if (g.dim == 3) {
+    z0 = cz - reach < 0 ? 0 : cz - reach;
+    z1 = cz + reach > g.n[2] - 1 ? g.n[2] - 1 : cz + reach;
+  }
+  constexpr int kMaxRuns = 27;
+  uint32_t cur[kMaxRuns], end[kMaxRuns];
+  int32_t head[kMaxRuns];
+  int nr = 0;
+  for (int z = z0; z <= z1; z++) {
+    for (int y = y0; y <= y1; y++) {
+      const uint32_t row = ((uint32_t) z * (uint32_t) g.n[1] + (uint32_t) y) * (uint32_t) g.n[0];
+      for (int x = x0; x <= x1 && nr < kMaxRuns; x++) {
+        uint32_t a = g.start[row + x], b = g.start[row + x + 1];
+        if (a < b) {
+          cur[nr] = a;
+          end[nr] = b;
+          head[nr] = ids[a];
+          nr++;
+        }
+      }
+    }
+  }
+  while (nr > 0) {
+    int best = 0;
+    for (int r = 1; r < nr; r++) best = head[r] < head[best] ? r : best;
+    const uint32_t j = cur[best];
+    visit(j);
+    cur[best] = j + 1;
+    if (j + 1 < end[best]) {
+      head[best] = ids[j + 1];
+    } else {
+      nr--;
+      cur[best] = cur[nr];
+      end[best] = end[nr];
+      head[best] = head[nr];
+    }
+  }
+}
+
+template <typename F>
+__device__ __forceinline__ int3 cell_of(const GridView &g, F x, F y, F z) {
+  int3 c;
+  c.x = cell_coord((double) x, g.lo[0], g.inv_cell, g.n[0]);
+  c.y = cell_coord((double) y, g.lo[1], g.inv_cell, g.n[1]);
+  c.z = g.dim == 3 ? cell_coord((double) z, g.lo[2], g.inv_cell, g.n[2]) : 0;
+  return c;
+}
+
+}  // namespace abl

@@ -1,0 +1,954 @@
+// abl_runtime.hip -- the -b hip runtime core: agent storage, the grid
+// pipeline, the simulate loop and the C ABI declared in include/abl_hip.h.
+//
+// Storage.  Each agent type keeps one device array per scalar component of
+// each member (struct-of-arrays; float2/float3 members split into x, y, z),
+// two buffers per column plus a per-column "current" index, and an int32 id
+// column (the agent's index in the reference's array).  A step function
+// writes only the columns it assigns into their other buffer and flips those;
+// columns it does not assign keep their values, which is the copy-through
+// semantics of the Mason and FlameGPU backends and, for every model whose
+// steps assign all members, of the C backend (src/backend/CPrinter.cpp:208-228).
+//
+// Grid pipeline (per near-loop agent type, rebuilt when its positions moved):
+//   bin      key[i] = cell(pos[i]); count[key]++            (atomics, L2)
+//   scan     start[c] = exclusive prefix of count           (wave64 scans)
+//   scatter  slot = start[c] + --count[c]; tmp[slot] = i    (count returns to 0)
+//   rank     stable placement inside each cell by id, fused with the gather
+//            of every column into the other buffer          (HBM-bound)
+// After the pipeline the agents are cell-sorted and, inside each cell, in
+// ascending id, which is independent of history: a run gives the same result
+// alone or beside other runs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/abl_hip.h"
+#include "abl_runtime.hpp"
+
+using namespace abl;
+
+// ---- errors -----------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      return fail(ABL_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                            \
+  } while (0)
+
+extern "C" const char *abl_hip_last_error(void) { return g_last_error.c_str(); }
+
+// ---- program registry -----------------------------------------------------
+
+static const ProgramDesc *const *programs() {
+  static const ProgramDesc *table[] = {
+      program_circle(), program_circle3d(), program_boids2d(), program_game_of_life(),
+      nullptr,
+  };
+  return table;
+}
+
+extern "C" int abl_hip_program_count(void) {
+  int n = 0;
+  while (programs()[n]) n++;
+  return n;
+}
+
+extern "C" const char *abl_hip_program_name(int index) {
+  if (index < 0 || index >= abl_hip_program_count()) return nullptr;
+  return programs()[index]->name;
+}
+
+// ---- storage ---------------------------------------------------------------
+
+namespace {
+
+struct Column {
+  int member;
+  int comp;
+  int elem;          // bytes per value: 1 (bool), 4 (int, float), 8 (double)
+  size_t aos_off;    // byte offset in the generated struct
+};
+
+struct AgentStore {
+  const AgentDesc *desc = nullptr;
+  std::vector<Column> cols;
+  size_t aos_size = 0;
+  int pos_col = -1;  // first position column
+  int dim = 0;
+  size_t n = 0, cap = 0;
+  void *mem[2][kMaxCols] = {};
+  int cur[kMaxCols] = {};
+  int32_t *id[2] = {nullptr, nullptr};
+  int id_cur = 0;
+  // grid
+  uint32_t *key = nullptr, *count = nullptr, *start = nullptr, *bsum = nullptr;
+  uint32_t *tmp_idx = nullptr;
+  int32_t *tmp_id = nullptr;
+  size_t ncells = 0, cells_cap = 0;
+  bool grid_valid = false;
+  GridView gv{};
+  double cell = 0.0;
+};
+
+}  // namespace
+
+struct abl_sim {
+  const ProgramDesc *prog = nullptr;
+  int precision = ABL_PREC_F64;
+  std::vector<double> params;
+  std::vector<AgentStore> types;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int block = 256;
+  int cell_scale = 1;
+  int order = 0;
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double last_ms[3] = {0, 0, 0};
+  // per-kernel timing ("hip.timing"): event pairs around every neighbour-step
+  // launch (kind 1) and grid build (kind 2) of the current simulate call
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> spans;
+  void *staging = nullptr;
+  size_t staging_cap = 0;
+  size_t footprint = 0;
+};
+
+// C struct layout of the generated agent struct (src/backend/CPrinter.cpp:250-253):
+// members in order, each aligned to its scalar size, total padded to the
+// largest alignment.
+static void layout_agent(AgentStore &s, const AgentDesc &d, int precision) {
+  s.desc = &d;
+  s.cols.clear();
+  size_t off = 0, align_max = 1;
+  const int fb = precision == ABL_PREC_F64 ? 8 : 4;
+  for (size_t m = 0; m < d.members.size(); m++) {
+    const MemberDesc &md = d.members[m];
+    int elem = md.kind == K_BOOL ? 1 : md.kind == K_INT ? 4 : fb;
+    size_t a = (size_t) elem;
+    off = (off + a - 1) / a * a;
+    int nc = kind_components(md.kind);
+    if (md.is_pos) {
+      s.pos_col = (int) s.cols.size();
+      s.dim = nc;
+    }
+    for (int c = 0; c < nc; c++) s.cols.push_back(Column{(int) m, c, elem, off + (size_t) c * elem});
+    off += (size_t) nc * elem;
+    align_max = std::max(align_max, a);
+  }
+  s.aos_size = (off + align_max - 1) / align_max * align_max;
+}
+
+static void free_store(AgentStore &s) {
+  for (int b = 0; b < 2; b++) {
+    for (int c = 0; c < kMaxCols; c++) {
+      if (s.mem[b][c]) (void) hipFree(s.mem[b][c]);
+      s.mem[b][c] = nullptr;
+    }
+    if (s.id[b]) (void) hipFree(s.id[b]);
+    s.id[b] = nullptr;
+  }
+  void *grid_ptrs[] = {s.key, s.count, s.start, s.bsum, s.tmp_idx, s.tmp_id};
+  for (void *p : grid_ptrs) {
+    if (p) (void) hipFree(p);
+  }
+  s.key = s.count = s.start = s.bsum = s.tmp_idx = nullptr;
+  s.tmp_id = nullptr;
+  s.n = s.cap = 0;
+  s.ncells = s.cells_cap = 0;
+  s.grid_valid = false;
+}
+
+static size_t store_bytes(const AgentStore &s) {
+  size_t b = 0;
+  for (const Column &c : s.cols) b += 2 * s.cap * (size_t) c.elem;
+  b += 2 * s.cap * 4;                      // ids
+  b += s.cap * (4 + 4 + 4);                // key, tmp_idx, tmp_id
+  b += s.cells_cap * 4 * 2 + 4096 * 4;     // count, start, bsum
+  return b;
+}
+
+static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
+  if (n <= s.cap) return ABL_OK;
+  size_t cap = std::max<size_t>(n, 1024);
+  for (int b = 0; b < 2; b++) {
+    for (size_t c = 0; c < s.cols.size(); c++) {
+      void *p = nullptr;
+      HIP_TRY(hipMalloc(&p, cap * (size_t) s.cols[c].elem));
+      if (s.mem[b][c] && s.n) {
+        HIP_TRY(hipMemcpyAsync(p, s.mem[b][c], s.n * (size_t) s.cols[c].elem,
+                               hipMemcpyDeviceToDevice, sim->stream));
+      }
+      if (s.mem[b][c]) {
+        HIP_TRY(hipStreamSynchronize(sim->stream));
+        HIP_TRY(hipFree(s.mem[b][c]));
+      }
+      s.mem[b][c] = p;
+    }
+    int32_t *p = nullptr;
+    HIP_TRY(hipMalloc(&p, cap * 4));
+    if (s.id[b] && s.n) {
+      HIP_TRY(hipMemcpyAsync(p, s.id[b], s.n * 4, hipMemcpyDeviceToDevice, sim->stream));
+    }
+    if (s.id[b]) {
+      HIP_TRY(hipStreamSynchronize(sim->stream));
+      HIP_TRY(hipFree(s.id[b]));
+    }
+    s.id[b] = p;
+  }
+  uint32_t *ptrs[2] = {s.key, s.tmp_idx};
+  for (uint32_t *p : ptrs) {
+    if (p) HIP_TRY(hipFree(p));
+  }
+  if (s.tmp_id) HIP_TRY(hipFree(s.tmp_id));
+  HIP_TRY(hipMalloc(&s.key, cap * 4));
+  HIP_TRY(hipMalloc(&s.tmp_idx, cap * 4));
+  HIP_TRY(hipMalloc(&s.tmp_id, cap * 4));
+  s.cap = cap;
+  return ABL_OK;
+}
+
+static int reserve_staging(abl_sim *sim, size_t bytes) {
+  if (bytes <= sim->staging_cap) return ABL_OK;
+  if (sim->staging) HIP_TRY(hipFree(sim->staging));
+  sim->staging = nullptr;
+  HIP_TRY(hipMalloc(&sim->staging, bytes));
+  sim->staging_cap = bytes;
+  return ABL_OK;
+}
+
+// ---- kernels: transposition ------------------------------------------------
+
+struct ColXfer {
+  void *col[kMaxCols];
+  uint32_t aos_off[kMaxCols];
+  int elem[kMaxCols];
+  int ncols;
+  uint32_t aos_size;
+};
+
+__global__ void k_aos_to_soa(const char *__restrict__ aos, ColXfer x, int32_t *__restrict__ id,
+                             int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const char *rec = aos + (size_t) i * x.aos_size;
+  for (int c = 0; c < x.ncols; c++) {
+    const char *p = rec + x.aos_off[c];
+    switch (x.elem[c]) {
+      case 1: ((uint8_t *) x.col[c])[i] = *(const uint8_t *) p; break;
+      case 4: ((uint32_t *) x.col[c])[i] = *(const uint32_t *) p; break;
+      default: ((uint64_t *) x.col[c])[i] = *(const uint64_t *) p; break;
+    }
+  }
+  id[i] = i;
+}
+
+// Write slot s back to record id[s] (un-permute to the reference's order).
+__global__ void k_soa_to_aos(char *__restrict__ aos, ColXfer x, const int32_t *__restrict__ id,
+                             int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  char *rec = aos + (size_t) id[s] * x.aos_size;
+  for (int c = 0; c < x.ncols; c++) {
+    char *p = rec + x.aos_off[c];
+    switch (x.elem[c]) {
+      case 1: *(uint8_t *) p = ((const uint8_t *) x.col[c])[s]; break;
+      case 4: *(uint32_t *) p = ((const uint32_t *) x.col[c])[s]; break;
+      default: *(uint64_t *) p = ((const uint64_t *) x.col[c])[s]; break;
+    }
+  }
+}
+
+// ---- kernels: grid pipeline ----------------------------------------------------
+
+template <typename P>
+__global__ void k_bin(const P *__restrict__ px, const P *__restrict__ py, const P *__restrict__ pz,
+                      GridView g, uint32_t *__restrict__ key, uint32_t *__restrict__ count, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int3 c = cell_of<P>(g, px[i], py[i], g.dim == 3 ? pz[i] : (P) 0);
+  uint32_t k = ((uint32_t) c.z * (uint32_t) g.n[1] + (uint32_t) c.y) * (uint32_t) g.n[0] + (uint32_t) c.x;
+  key[i] = k;
+  atomicAdd(&count[k], 1u);
+}
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// Per tile: exclusive scan of count into start, tile total into bsum.
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *__restrict__ count,
+                                                          uint32_t *__restrict__ start,
+                                                          uint32_t *__restrict__ bsum, int ncells) {
+  __shared__ uint32_t lds[kWave + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = base + k < ncells ? count[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan(sum, lds, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < ncells) start[base + k] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile totals (one block, any number of tiles).
+__global__ __launch_bounds__(1024) void k_scan_totals(uint32_t *__restrict__ bsum, int ntiles,
+                                                     uint32_t *__restrict__ start, int ncells) {
+  __shared__ uint32_t lds[kWave + 1];
+  uint32_t carry = 0;
+  for (int base = 0; base < ntiles; base += blockDim.x) {
+    const int t = base + threadIdx.x;
+    uint32_t v = t < ntiles ? bsum[t] : 0u;
+    uint32_t total;
+    uint32_t pre = block_excl_scan(v, lds, &total);
+    if (t < ntiles) bsum[t] = pre + carry;
+    carry += total;
+  }
+  if (threadIdx.x == 0) start[ncells] = carry;
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ start, const uint32_t *__restrict__ bsum,
+                           int ncells) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) start[c] += bsum[c / kScanTile];
+}
+
+__global__ void k_scatter(const uint32_t *__restrict__ key, uint32_t *__restrict__ count,
+                          const uint32_t *__restrict__ start, const int32_t *__restrict__ id,
+                          uint32_t *__restrict__ tmp_idx, int32_t *__restrict__ tmp_id, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = key[i];
+  const uint32_t slot = start[k] + atomicSub(&count[k], 1u) - 1u;
+  tmp_idx[slot] = (uint32_t) i;
+  tmp_id[slot] = id[i];
+}
+
+struct GatherCols {
+  const void *src[kMaxCols];
+  void *dst[kMaxCols];
+  int elem[kMaxCols];
+  int ncols;
+};
+
+// Stable placement inside each cell by id, fused with the column gather.
+__global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const int32_t *__restrict__ tmp_id,
+                              const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
+                              GatherCols g, int32_t *__restrict__ id_out, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t src = tmp_idx[s];
+  const int32_t my = tmp_id[s];
+  const uint32_t k = key[src];
+  const uint32_t a = start[k], b = start[k + 1];
+  uint32_t rank = 0;
+  for (uint32_t t = a; t < b; t++) rank += tmp_id[t] < my ? 1u : 0u;
+  const uint32_t dst = a + rank;
+  id_out[dst] = my;
+  for (int c = 0; c < g.ncols; c++) {
+    switch (g.elem[c]) {
+      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
+      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
+      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
+    }
+  }
+}
+
+// ---- kernels: neighbour-set probe (parity) -----------------------------------
+
+template <typename P>
+__global__ void k_probe(const P *__restrict__ ax, const P *__restrict__ ay, const P *__restrict__ az,
+                        const int32_t *__restrict__ aid, int n, const P *__restrict__ bx,
+                        const P *__restrict__ by, const P *__restrict__ bz,
+                        const int32_t *__restrict__ bid, GridView g, float t,
+                        uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
+                        int64_t *__restrict__ pairs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const P x = ax[i], y = ay[i], z = g.dim == 3 ? az[i] : (P) 0;
+  const int3 c = cell_of<P>(g, x, y, z);
+  uint32_t cnt = 0;
+  uint32_t w = offsets ? offsets[i] : 0u;
+  for_each_candidate(g, c.x, c.y, c.z, g.reach, [&](uint32_t j) {
+    float f = g.dim == 3 ? dist_sq_f32<P>(bx[j], by[j], bz[j], x, y, z)
+                         : dist_sq_f32<P>(bx[j], by[j], x, y);
+    if (f > t) return;
+    if (pairs) {
+      pairs[2 * (size_t) w] = aid[i];
+      pairs[2 * (size_t) w + 1] = bid[j];
+      w++;
+    }
+    cnt++;
+  });
+  if (counts) counts[i] = cnt;
+}
+
+// ---- grid build ------------------------------------------------------------------
+
+static void *col_cur(const AgentStore &s, int c) { return s.mem[s.cur[c]][c]; }
+static void *col_other(const AgentStore &s, int c) { return s.mem[1 - s.cur[c]][c]; }
+
+static int setup_grid(abl_sim *sim, AgentStore &s) {
+  double lo[3], hi[3], gran;
+  sim->prog->env(sim->params.data(), lo, hi, &gran);
+  if (!(gran > 0.0)) return fail(ABL_ERR_ARG, "environment granularity must be positive");
+  const double cell = gran / (double) sim->cell_scale * (1.0 + 1.0 / 1048576.0);
+  size_t ncells = 1;
+  GridView g{};
+  g.dim = s.dim;
+  for (int d = 0; d < 3; d++) {
+    g.lo[d] = d < s.dim ? lo[d] : 0.0;
+    int nd = 1;
+    if (d < s.dim) {
+      double span = hi[d] - lo[d];
+      double q = std::ceil(span / cell);
+      nd = q < 1.0 ? 1 : (int) q;
+    }
+    g.n[d] = nd;
+    ncells *= (size_t) nd;
+  }
+  if (ncells > (size_t) 1 << 31) return fail(ABL_ERR_MEMORY, "grid has too many cells");
+  g.inv_cell = 1.0 / cell;
+  s.cell = cell;
+  if (ncells + 1 > s.cells_cap) {
+    if (s.count) HIP_TRY(hipFree(s.count));
+    if (s.start) HIP_TRY(hipFree(s.start));
+    if (s.bsum) HIP_TRY(hipFree(s.bsum));
+    s.count = s.start = s.bsum = nullptr;
+    HIP_TRY(hipMalloc(&s.count, (ncells + 1) * 4));
+    HIP_TRY(hipMalloc(&s.start, (ncells + 1) * 4));
+    size_t ntiles = (ncells + kScanTile - 1) / kScanTile;
+    HIP_TRY(hipMalloc(&s.bsum, std::max<size_t>(ntiles, 1) * 4));
+    HIP_TRY(hipMemsetAsync(s.count, 0, (ncells + 1) * 4, sim->stream));
+    s.cells_cap = ncells + 1;
+  }
+  s.ncells = ncells;
+  g.start = s.start;
+  s.gv = g;
+  return ABL_OK;
+}
+
+static int build_grid(abl_sim *sim, AgentStore &s) {
+  if (s.pos_col < 0) return fail(ABL_ERR_ARG, "near loop over an agent type without position");
+  int rc = setup_grid(sim, s);
+  if (rc) return rc;
+  const int n = (int) s.n;
+  const GridView &g = s.gv;
+  const int nc = (int) s.ncells;
+  hipStream_t st = sim->stream;
+  const int blk = 256;
+  if (n > 0) {
+    const int pc = s.pos_col;
+    if (s.cols[pc].elem == 8) {
+      hipLaunchKernelGGL(k_bin<double>, grid_blocks(n, blk), blk, 0, st,
+                         (const double *) col_cur(s, pc), (const double *) col_cur(s, pc + 1),
+                         (const double *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr), g, s.key,
+                         s.count, n);
+    } else {
+      hipLaunchKernelGGL(k_bin<float>, grid_blocks(n, blk), blk, 0, st,
+                         (const float *) col_cur(s, pc), (const float *) col_cur(s, pc + 1),
+                         (const float *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr), g, s.key,
+                         s.count, n);
+    }
+  }
+  const int ntiles = (nc + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles, ntiles, kScanBlock, 0, st, s.count, s.start, s.bsum, nc);
+  hipLaunchKernelGGL(k_scan_totals, 1, 1024, 0, st, s.bsum, ntiles, s.start, nc);
+  hipLaunchKernelGGL(k_scan_add, grid_blocks(nc, 256), 256, 0, st, s.start, s.bsum, nc);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, grid_blocks(n, blk), blk, 0, st, s.key, s.count, s.start,
+                       s.id[s.id_cur], s.tmp_idx, s.tmp_id, n);
+    GatherCols gc{};
+    gc.ncols = (int) s.cols.size();
+    for (int c = 0; c < gc.ncols; c++) {
+      gc.src[c] = col_cur(s, c);
+      gc.dst[c] = col_other(s, c);
+      gc.elem[c] = s.cols[c].elem;
+    }
+    hipLaunchKernelGGL(k_rank_gather, grid_blocks(n, blk), blk, 0, st, s.tmp_idx, s.tmp_id,
+                       s.key, s.start, gc, s.id[1 - s.id_cur], n);
+    for (int c = 0; c < gc.ncols; c++) s.cur[c] ^= 1;
+    s.id_cur ^= 1;
+  }
+  HIP_TRY(hipGetLastError());
+  s.grid_valid = true;
+  return ABL_OK;
+}
+
+static hipEvent_t take_event(abl_sim *sim) {
+  if (sim->evused == sim->evpool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    sim->evpool.push_back(e);
+  }
+  return sim->evpool[sim->evused++];
+}
+
+// Record the start of a timed span; returns the index to close it with.
+static int span_open(abl_sim *sim, int kind) {
+  if (!sim->timing) return -1;
+  hipEvent_t a = take_event(sim), b = take_event(sim);
+  if (!a || !b) return -1;
+  (void) hipEventRecord(a, sim->stream);
+  sim->spans.push_back({kind, {a, b}});
+  return (int) sim->spans.size() - 1;
+}
+
+static void span_close(abl_sim *sim, int idx) {
+  if (idx >= 0) (void) hipEventRecord(sim->spans[idx].second.second, sim->stream);
+}
+
+static int reach_for(const AgentStore &s, double radius) {
+  double q = radius / s.cell * (1.0 + 1.0 / 2097152.0);
+  int r = (int) std::ceil(q);
+  return r < 1 ? 1 : r;
+}
+
+static double step_radius(const abl_sim *sim, const StepDesc &st) {
+  for (size_t p = 0; p < sim->prog->params.size(); p++) {
+    if (std::strcmp(sim->prog->params[p].name, st.radius_param) == 0) {
+      // under use_float the parameter is a float global, widened for the
+      // double-literal product the near radius expression prints as
+      double v = sim->params[p];
+      if (sim->precision != ABL_PREC_F64) v = (double) (float) v;
+      return st.radius_scale * v;
+    }
+  }
+  return 0.0;
+}
+
+// ---- C ABI -----------------------------------------------------------------------
+
+#define CHECK_SIM(sim) \
+  if (!(sim)) return fail(ABL_ERR_ARG, "null simulation handle")
+
+extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out) {
+  if (!program || !out) return fail(ABL_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (precision != ABL_PREC_F64 && precision != ABL_PREC_F32)
+    return fail(ABL_ERR_UNSUPPORTED, "precision not supported by this build");
+  const ProgramDesc *prog = nullptr;
+  for (int i = 0; programs()[i]; i++) {
+    if (std::strcmp(programs()[i]->name, program) == 0) prog = programs()[i];
+  }
+  if (!prog) return fail(ABL_ERR_ARG, std::string("unknown program ") + program);
+  abl_sim *sim = new abl_sim();
+  sim->prog = prog;
+  sim->precision = precision;
+  for (const ParamDesc &p : prog->params) sim->params.push_back(p.value);
+  sim->types.resize(prog->agents.size());
+  for (size_t t = 0; t < prog->agents.size(); t++) {
+    layout_agent(sim->types[t], prog->agents[t], precision);
+    if (sim->types[t].cols.size() > (size_t) kMaxCols) {
+      delete sim;
+      return fail(ABL_ERR_UNSUPPORTED, "agent type has too many member components");
+    }
+  }
+  hipError_t e = hipGetDevice(&sim->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking);
+  for (int k = 0; k < 6 && e == hipSuccess; k++) e = hipEventCreate(&sim->ev[k]);
+  if (e != hipSuccess) {
+    std::string msg = std::string("stream/event creation: ") + hipGetErrorString(e);
+    delete sim;
+    return fail(ABL_ERR_DEVICE, msg);
+  }
+  *out = sim;
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_destroy(abl_sim *sim) {
+  CHECK_SIM(sim);
+  (void) hipStreamSynchronize(sim->stream);
+  for (AgentStore &s : sim->types) free_store(s);
+  if (sim->staging) (void) hipFree(sim->staging);
+  for (int k = 0; k < 6; k++) {
+    if (sim->ev[k]) (void) hipEventDestroy(sim->ev[k]);
+  }
+  for (hipEvent_t e : sim->evpool) (void) hipEventDestroy(e);
+  if (sim->stream) (void) hipStreamDestroy(sim->stream);
+  delete sim;
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_set_param(abl_sim *sim, const char *name, double value) {
+  CHECK_SIM(sim);
+  if (!name) return fail(ABL_ERR_ARG, "null parameter name");
+  for (size_t p = 0; p < sim->prog->params.size(); p++) {
+    if (std::strcmp(sim->prog->params[p].name, name) == 0) {
+      sim->params[p] = value;
+      for (AgentStore &s : sim->types) s.grid_valid = false;
+      return ABL_OK;
+    }
+  }
+  return fail(ABL_ERR_ARG, std::string("Unknown parameter \"") + name + "\"");
+}
+
+extern "C" int abl_hip_get_param(abl_sim *sim, const char *name, double *value) {
+  CHECK_SIM(sim);
+  if (!name || !value) return fail(ABL_ERR_ARG, "null argument");
+  for (size_t p = 0; p < sim->prog->params.size(); p++) {
+    if (std::strcmp(sim->prog->params[p].name, name) == 0) {
+      *value = sim->params[p];
+      return ABL_OK;
+    }
+  }
+  return fail(ABL_ERR_ARG, std::string("Unknown parameter \"") + name + "\"");
+}
+
+extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *value) {
+  CHECK_SIM(sim);
+  if (!key || !value) return fail(ABL_ERR_ARG, "null argument");
+  std::string k(key), v(value);
+  if (k == "hip.block") {
+    int b = std::atoi(value);
+    if (b < 64 || b > 1024 || b % 64) return fail(ABL_ERR_ARG, "hip.block must be 64..1024, a multiple of 64");
+    sim->block = b;
+  } else if (k == "hip.cell_scale") {
+    int c = std::atoi(value);
+    if (c < 1 || c > 4) return fail(ABL_ERR_ARG, "hip.cell_scale must be 1..4");
+    sim->cell_scale = c;
+    for (AgentStore &s : sim->types) s.grid_valid = false;
+  } else if (k == "hip.order") {
+    if (v == "cell") sim->order = 0;
+    else if (v == "index") sim->order = 1;
+    else return fail(ABL_ERR_ARG, "hip.order must be cell or index");
+  } else if (k == "hip.timing") {
+    sim->timing = v == "1" || v == "true";
+  } else {
+    return fail(ABL_ERR_ARG, "unknown config key " + k);
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_agent_type_count(abl_sim *sim) {
+  if (!sim) return fail(-1, "null simulation handle");
+  return (int) sim->types.size();
+}
+
+extern "C" const char *abl_hip_agent_type_name(abl_sim *sim, int t) {
+  if (!sim || t < 0 || t >= (int) sim->types.size()) return nullptr;
+  return sim->types[t].desc->name;
+}
+
+extern "C" size_t abl_hip_agent_size(abl_sim *sim, int t) {
+  if (!sim || t < 0 || t >= (int) sim->types.size()) return 0;
+  return sim->types[t].aos_size;
+}
+
+static ColXfer xfer_of(const AgentStore &s, bool current) {
+  ColXfer x{};
+  x.ncols = (int) s.cols.size();
+  x.aos_size = (uint32_t) s.aos_size;
+  for (int c = 0; c < x.ncols; c++) {
+    x.col[c] = current ? col_cur(s, c) : col_other(s, c);
+    x.aos_off[c] = (uint32_t) s.cols[c].aos_off;
+    x.elem[c] = s.cols[c].elem;
+  }
+  return x;
+}
+
+extern "C" int abl_hip_upload(abl_sim *sim, int t, const void *aos, size_t count) {
+  CHECK_SIM(sim);
+  if (t < 0 || t >= (int) sim->types.size()) return fail(ABL_ERR_ARG, "agent type out of range");
+  if (count && !aos) return fail(ABL_ERR_ARG, "null agent data");
+  if (count > (size_t) INT32_MAX) return fail(ABL_ERR_ARG, "too many agents");
+  AgentStore &s = sim->types[t];
+  int rc = reserve_store(sim, s, count);
+  if (rc) return rc;
+  s.n = count;
+  s.grid_valid = false;
+  if (count == 0) return ABL_OK;
+  rc = reserve_staging(sim, count * s.aos_size);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(sim->staging, aos, count * s.aos_size, hipMemcpyHostToDevice, sim->stream));
+  ColXfer x = xfer_of(s, true);
+  hipLaunchKernelGGL(k_aos_to_soa, grid_blocks((int) count, 256), 256, 0, sim->stream,
+                     (const char *) sim->staging, x, s.id[s.id_cur], (int) count);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_count(abl_sim *sim, int t, size_t *count) {
+  CHECK_SIM(sim);
+  if (t < 0 || t >= (int) sim->types.size() || !count) return fail(ABL_ERR_ARG, "bad argument");
+  *count = sim->types[t].n;
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_download(abl_sim *sim, int t, void *aos, size_t capacity, size_t *count) {
+  CHECK_SIM(sim);
+  if (t < 0 || t >= (int) sim->types.size()) return fail(ABL_ERR_ARG, "agent type out of range");
+  AgentStore &s = sim->types[t];
+  if (count) *count = s.n;
+  if (s.n == 0) return ABL_OK;
+  if (!aos || capacity < s.n) return fail(ABL_ERR_ARG, "download buffer too small");
+  int rc = reserve_staging(sim, s.n * s.aos_size);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(sim->staging, 0, s.n * s.aos_size, sim->stream));
+  ColXfer x = xfer_of(s, true);
+  hipLaunchKernelGGL(k_soa_to_aos, grid_blocks((int) s.n, 256), 256, 0, sim->stream,
+                     (char *) sim->staging, x, s.id[s.id_cur], (int) s.n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(aos, sim->staging, s.n * s.aos_size, hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  return ABL_OK;
+}
+
+static int agent_index(abl_sim *sim, const char *name) {
+  for (size_t t = 0; t < sim->types.size(); t++) {
+    if (std::strcmp(sim->types[t].desc->name, name) == 0) return (int) t;
+  }
+  return -1;
+}
+
+extern "C" int abl_hip_upload_all(abl_sim *sim, const void *agents, const abl_agent_info *info) {
+  CHECK_SIM(sim);
+  if (!agents || !info) return fail(ABL_ERR_ARG, "null argument");
+  for (const abl_agent_info *a = info; a->name; a++) {
+    int t = agent_index(sim, a->name);
+    if (t < 0) return fail(ABL_ERR_ARG, std::string("unknown agent type ") + a->name);
+    const abl_type_info *end = a->info;
+    while (end->type != ABL_TYPE_END) end++;
+    if (end->offset != sim->types[t].aos_size)
+      return fail(ABL_ERR_ARG, std::string("struct size mismatch for ") + a->name);
+    const abl_dyn_array *arr = (const abl_dyn_array *) ((const char *) agents + a->offset);
+    int rc = abl_hip_upload(sim, t, arr->values, arr->len);
+    if (rc) return rc;
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_download_all(abl_sim *sim, void *agents, const abl_agent_info *info) {
+  CHECK_SIM(sim);
+  if (!agents || !info) return fail(ABL_ERR_ARG, "null argument");
+  for (const abl_agent_info *a = info; a->name; a++) {
+    int t = agent_index(sim, a->name);
+    if (t < 0) return fail(ABL_ERR_ARG, std::string("unknown agent type ") + a->name);
+    abl_dyn_array *arr = (abl_dyn_array *) ((char *) agents + a->offset);
+    size_t n = sim->types[t].n;
+    if (arr->cap < n) {
+      void *p = std::realloc(arr->values, n * sim->types[t].aos_size);
+      if (!p) return fail(ABL_ERR_MEMORY, "host allocation failed");
+      arr->values = p;
+      arr->cap = n;
+    }
+    int rc = abl_hip_download(sim, t, arr->values, arr->cap, &arr->len);
+    if (rc) return rc;
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_init_agents(abl_sim *sim) {
+  CHECK_SIM(sim);
+  if (!sim->prog->host_init) return fail(ABL_ERR_UNSUPPORTED, "program has no host initialisation");
+  return sim->prog->host_init(sim, sim->params.data(), sim->precision);
+}
+
+static int launch_step(abl_sim *sim, const StepDesc &st) {
+  AgentStore &a = sim->types[st.agent];
+  StepLaunch L{};
+  L.stream = sim->stream;
+  L.precision = sim->precision;
+  L.n = (int) a.n;
+  L.params = sim->params.data();
+  L.block = sim->block;
+  L.order = sim->order;
+  for (size_t c = 0; c < a.cols.size(); c++) {
+    L.in.c[c] = col_cur(a, (int) c);
+    L.out.c[c] = (st.writes >> c) & 1u ? col_other(a, (int) c) : nullptr;
+  }
+  if (st.near_agent >= 0) {
+    AgentStore &b = sim->types[st.near_agent];
+    if (!b.grid_valid) {
+      int sp = span_open(sim, 2);
+      int rc = build_grid(sim, b);
+      span_close(sim, sp);
+      if (rc) return rc;
+      if (st.near_agent == st.agent) {
+        for (size_t c = 0; c < a.cols.size(); c++) {
+          L.in.c[c] = col_cur(a, (int) c);
+          L.out.c[c] = (st.writes >> c) & 1u ? col_other(a, (int) c) : nullptr;
+        }
+      }
+    }
+    const double r = step_radius(sim, st);
+    L.nb_n = (int) b.n;
+    for (size_t c = 0; c < b.cols.size(); c++) L.nb.c[c] = col_cur(b, (int) c);
+    L.nb_id = b.id[b.id_cur];
+    L.grid = b.gv;
+    L.grid.reach = reach_for(b, r);
+    L.near_t = sqrtf_le_threshold(r);
+    if (sim->order == 1) {
+      int side = 2 * L.grid.reach + 1;
+      int runs = side * side * (b.dim == 3 ? side : 1);
+      if (runs > 27) return fail(ABL_ERR_UNSUPPORTED, "hip.order=index needs a stencil of at most 27 cells");
+    }
+  }
+  if (L.n > 0) {
+    int sp = span_open(sim, 1);
+    st.launch(L);
+    span_close(sim, sp);
+    HIP_TRY(hipGetLastError());
+  }
+  uint32_t w = st.writes;
+  for (size_t c = 0; c < a.cols.size(); c++) {
+    if ((w >> c) & 1u) a.cur[c] ^= 1;
+  }
+  if (a.pos_col >= 0) {
+    for (int d = 0; d < a.dim; d++) {
+      if ((w >> (a.pos_col + d)) & 1u) a.grid_valid = false;
+    }
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_simulate(abl_sim *sim, int timesteps) {
+  CHECK_SIM(sim);
+  if (timesteps < 0) return fail(ABL_ERR_ARG, "negative timestep count");
+  sim->evused = 0;
+  sim->spans.clear();
+  if (sim->timing) HIP_TRY(hipEventRecord(sim->ev[0], sim->stream));
+  for (int t = 0; t < timesteps; t++) {
+    for (const StepDesc &st : sim->prog->steps) {
+      int rc = launch_step(sim, st);
+      if (rc) return rc;
+    }
+  }
+  if (sim->timing) {
+    HIP_TRY(hipEventRecord(sim->ev[1], sim->stream));
+    HIP_TRY(hipEventSynchronize(sim->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, sim->ev[0], sim->ev[1]));
+    sim->last_ms[0] = ms;
+    sim->last_ms[1] = sim->last_ms[2] = 0.0;
+    for (auto &sp : sim->spans) {
+      float m = 0.f;
+      HIP_TRY(hipEventElapsedTime(&m, sp.second.first, sp.second.second));
+      sim->last_ms[sp.first] += m;
+    }
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_synchronize(abl_sim *sim) {
+  CHECK_SIM(sim);
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_footprint(abl_sim *sim, size_t *bytes) {
+  CHECK_SIM(sim);
+  if (!bytes) return fail(ABL_ERR_ARG, "null argument");
+  size_t b = sim->staging_cap;
+  for (const AgentStore &s : sim->types) b += store_bytes(s);
+  *bytes = b;
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_last_timing(abl_sim *sim, double out_ms[3]) {
+  CHECK_SIM(sim);
+  if (!out_ms) return fail(ABL_ERR_ARG, "null argument");
+  for (int k = 0; k < 3; k++) out_ms[k] = sim->last_ms[k];
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t capacity,
+                                  size_t *count) {
+  CHECK_SIM(sim);
+  if (step < 0 || step >= (int) sim->prog->steps.size()) return fail(ABL_ERR_ARG, "step out of range");
+  const StepDesc &st = sim->prog->steps[step];
+  if (count) *count = 0;
+  if (st.near_agent < 0) return ABL_OK;
+  AgentStore &a = sim->types[st.agent];
+  AgentStore &b = sim->types[st.near_agent];
+  if (!b.grid_valid) {
+    int rc = build_grid(sim, b);
+    if (rc) return rc;
+  }
+  const int n = (int) a.n;
+  if (n == 0) return ABL_OK;
+  const double r = step_radius(sim, st);
+  GridView g = b.gv;
+  g.reach = reach_for(b, r);
+  const float t = sqrtf_le_threshold(r);
+  uint32_t *cnt = nullptr, *off = nullptr;
+  int64_t *dp = nullptr;
+  HIP_TRY(hipMalloc(&cnt, (size_t) n * 4));
+  HIP_TRY(hipMalloc(&off, (size_t) n * 4));
+  const int pa = a.pos_col, pb = b.pos_col;
+  auto launch = [&](uint32_t *counts, const uint32_t *offsets, int64_t *out) {
+    if (a.cols[pa].elem == 8) {
+      hipLaunchKernelGGL(k_probe<double>, grid_blocks(n, 256), 256, 0, sim->stream,
+                         (const double *) col_cur(a, pa), (const double *) col_cur(a, pa + 1),
+                         (const double *) (a.dim == 3 ? col_cur(a, pa + 2) : nullptr), a.id[a.id_cur], n,
+                         (const double *) col_cur(b, pb), (const double *) col_cur(b, pb + 1),
+                         (const double *) (b.dim == 3 ? col_cur(b, pb + 2) : nullptr), b.id[b.id_cur], g,
+                         t, counts, offsets, out);
+    } else {
+      hipLaunchKernelGGL(k_probe<float>, grid_blocks(n, 256), 256, 0, sim->stream,
+                         (const float *) col_cur(a, pa), (const float *) col_cur(a, pa + 1),
+                         (const float *) (a.dim == 3 ? col_cur(a, pa + 2) : nullptr), a.id[a.id_cur], n,
+                         (const float *) col_cur(b, pb), (const float *) col_cur(b, pb + 1),
+                         (const float *) (b.dim == 3 ? col_cur(b, pb + 2) : nullptr), b.id[b.id_cur], g,
+                         t, counts, offsets, out);
+    }
+  };
+  launch(cnt, nullptr, nullptr);
+  std::vector<uint32_t> hc((size_t) n);
+  HIP_TRY(hipMemcpyAsync(hc.data(), cnt, (size_t) n * 4, hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  std::vector<uint32_t> ho((size_t) n);
+  size_t total = 0;
+  for (int i = 0; i < n; i++) {
+    ho[i] = (uint32_t) total;
+    total += hc[i];
+  }
+  if (count) *count = total;
+  if (pairs && total > 0) {
+    HIP_TRY(hipMemcpyAsync(off, ho.data(), (size_t) n * 4, hipMemcpyHostToDevice, sim->stream));
+    HIP_TRY(hipMalloc(&dp, total * 16));
+    launch(nullptr, off, dp);
+    std::vector<int64_t> hp(total * 2);
+    HIP_TRY(hipMemcpyAsync(hp.data(), dp, total * 16, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    std::vector<std::pair<int64_t, int64_t>> v(total);
+    for (size_t k = 0; k < total; k++) v[k] = {hp[2 * k], hp[2 * k + 1]};
+    std::sort(v.begin(), v.end());
+    size_t m = std::min(total, capacity);
+    for (size_t k = 0; k < m; k++) {
+      pairs[2 * k] = v[k].first;
+      pairs[2 * k + 1] = v[k].second;
+    }
+    HIP_TRY(hipFree(dp));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipFree(cnt));
+  HIP_TRY(hipFree(off));
+  return ABL_OK;
+}

@@ -1,0 +1,123 @@
+// abl_runtime.hpp -- internal interface between the runtime core
+// (abl_runtime.hip) and the step programs (prog_*.hip).
+//
+// A step program is what `-b hip` emits for one .abl model: its agent
+// types, parameters, environment, and one device kernel per step function.
+// The runtime owns agent storage (struct-of-arrays, two buffers per column),
+// the grid pipeline and the simulate loop; a program only launches kernels on
+// the column pointers the runtime hands it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "abl_device.hpp"
+
+struct abl_sim;
+
+namespace abl {
+
+enum MemberKind { K_BOOL = 1, K_INT = 2, K_FLOAT = 3, K_FLOAT2 = 5, K_FLOAT3 = 6 };
+
+inline int kind_components(int kind) {
+  return kind == K_FLOAT2 ? 2 : kind == K_FLOAT3 ? 3 : 1;
+}
+
+struct MemberDesc {
+  const char *name;
+  int kind;
+  bool is_pos;
+};
+
+struct AgentDesc {
+  const char *name;
+  std::vector<MemberDesc> members;
+};
+
+struct ParamDesc {
+  const char *name;
+  double value;  // default, as the reference prints it
+};
+
+struct StepLaunch;
+
+struct StepDesc {
+  const char *name;
+  int agent;                 // stepping agent type
+  int near_agent;            // agent type of the near loop, -1 if none
+  const char *radius_param;  // near radius = radius_scale * param (C double math)
+  double radius_scale;
+  uint32_t writes;           // bit per column index of `agent` the step writes
+  void (*launch)(const StepLaunch &);
+};
+
+struct ProgramDesc {
+  const char *name;
+  std::vector<AgentDesc> agents;
+  std::vector<ParamDesc> params;
+  std::vector<StepDesc> steps;
+  // environment after folding (src/AnalysisVisitor.cpp:474-541): min/max per
+  // axis and the granularity, as functions of the parameters.
+  void (*env)(const double *params, double lo[3], double hi[3], double *granularity);
+  // main() statements before simulate(...): builds every agent type's
+  // records on the host with the runtime's generator and uploads them.
+  int (*host_init)(struct ::abl_sim *sim, const double *params, int precision);
+};
+
+// Column pointers of one agent type (members in declaration order, vector
+// components expanded x, y, z).
+struct ColSet {
+  void *c[kMaxCols];
+};
+
+struct StepLaunch {
+  hipStream_t stream;
+  int precision;       // abl_precision
+  int n;               // agents of the stepping type
+  ColSet in;           // current values of the stepping type
+  ColSet out;          // destination of written columns (others: nullptr)
+  int nb_n;            // agents of the near type
+  ColSet nb;           // near type, cell-sorted
+  const int32_t *nb_id;
+  GridView grid;
+  float near_t;        // T = max{f : sqrtf(f) <= radius}
+  const double *params;  // host copy, indexed as ProgramDesc::params
+  int block;           // threads per block
+  int order;           // 0 = cell order, 1 = reference index order
+};
+
+// Host-side helpers (abl_host.cpp).
+float sqrtf_le_threshold(double r);  // max float f >= 0 with sqrtf(f) <= r
+float sqrtf_lt_threshold(double r);  // min float f >= 0 with sqrtf(f) >= r
+
+// Program registry (one entry per prog_*.hip).
+const ProgramDesc *program_circle();
+const ProgramDesc *program_circle3d();
+const ProgramDesc *program_boids2d();
+const ProgramDesc *program_game_of_life();
+
+inline double param_of(const StepLaunch &L, int idx) { return L.params[idx]; }
+
+inline unsigned grid_blocks(int n, int block) {
+  return (unsigned) ((n + block - 1) / block);
+}
+
+}  // namespace abl
+
+#include "../../include/abl_hip.h"
+
+namespace abl {
+
+// The generator in the build's float type: random_float of asset/c/libabl.c
+// compiled with abl_float = double, or float under use_float.
+template <typename F>
+inline F host_random(F lo, F hi);
+template <>
+inline double host_random<double>(double lo, double hi) { return abl_random_float(lo, hi); }
+template <>
+inline float host_random<float>(float lo, float hi) { return abl_random_floatf(lo, hi); }
+
+}  // namespace abl

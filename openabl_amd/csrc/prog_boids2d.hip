@@ -1,0 +1,206 @@
+// prog_boids2d.hip -- step program for examples/boids2d_cfg.abl (and, with
+// steer/collision/match = 0.65/0.75/1.25 and max_speed = 1, for
+// examples/boids2d.abl): the device code `-b hip` emits for update_boid.
+//
+// Per agent (one thread, cell-sorted order), operation for operation as the
+// C backend prints it (src/backend/CPrinter.cpp:148-173,
+// GenericCPrinter.cpp:20-75):
+//   near loop over dist(nx.pos, in.pos) <= interaction_radius:
+//     global_center += nx.pos; global_velocity += nx.velocity; count += 1
+//     if (dist(in.pos, nx.pos) < separation_radius) collision_center += nx.pos, ...
+//   steer/match/avoid terms, new_velocity = velocity + change * global_scale,
+//   speed clamp v / |v| * max_speed, new_pos = pos + v * time_scale,
+//   boundPosition wrap.
+// Both distance comparisons are done on the float-rounded square sum against
+// host-computed thresholds (exact: sqrtf is monotone), so the loop takes no
+// square root; only |new_velocity| needs one.
+#include <vector>
+
+#include "abl_runtime.hpp"
+
+namespace abl {
+namespace {
+
+enum { P_INTERACTION_RADIUS, P_SEPARATION_RADIUS, P_AGENT_DENSITY, P_TIME_SCALE, P_GLOBAL_SCALE,
+       P_STEER_SCALE, P_COLLISION_SCALE, P_MATCH_SCALE, P_MAX_SPEED, P_NUM_AGENTS,
+       P_NUM_TIMESTEPS, P_MIN_POS, P_MAX_POS };
+
+template <typename F>
+struct BoidArgs {
+  const F *px, *py, *vx, *vy;  // in (cell-sorted)
+  F *opx, *opy, *ovx, *ovy;    // out
+  const int32_t *id;
+  GridView g;
+  float near_t;   // dist <= interaction_radius  <=>  f <= near_t
+  float sep_u;    // dist <  separation_radius   <=>  f <  sep_u
+  int n;
+  F time_scale, global_scale, steer, collision, match, max_speed, min_pos, max_pos;
+};
+
+template <typename F, bool BY_ID>
+__global__ __launch_bounds__(1024) void boids_step(BoidArgs<F> a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  const F x = a.px[i], y = a.py[i];
+  const F vx = a.vx[i], vy = a.vy[i];
+  const int3 c = cell_of<F>(a.g, x, y, (F) 0);
+
+  F gvx = 0, gvy = 0, gcx = 0, gcy = 0, ccx = 0, ccy = 0;
+  int icount = 0, ccount = 0;
+  auto body = [&](uint32_t j) {
+    const F qx = a.px[j], qy = a.py[j];
+    const float f = dist_sq_f32<F>(qx, qy, x, y);
+    if (f > a.near_t) return;
+    gcx = gcx + qx;
+    gcy = gcy + qy;
+    gvx = gvx + a.vx[j];
+    gvy = gvy + a.vy[j];
+    icount += 1;
+    if (f < a.sep_u) {
+      ccx = ccx + qx;
+      ccy = ccy + qy;
+      ccount += 1;
+    }
+  };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, 0, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, 0, a.g.reach, body);
+  }
+
+  F chx = 0, chy = 0;  // velocity_change
+  F sx = 0, sy = 0;    // steer_velocity
+  if (icount > 0) {
+    gcx = gcx / (F) icount;
+    gcy = gcy / (F) icount;
+    sx = (gcx - x) * a.steer;
+    sy = (gcy - y) * a.steer;
+  }
+  chx = chx + sx;
+  chy = chy + sy;
+  F mx = 0, my = 0;    // match_velocity
+  if (icount > 0) {
+    gvx = gvx / (F) icount;
+    gvy = gvy / (F) icount;
+    mx = gvx * a.match;
+    my = gvy * a.match;
+  }
+  chx = chx + mx;
+  chy = chy + my;
+  F ax = 0, ay = 0;    // avoid_velocity
+  if (ccount > 0) {
+    ccx = ccx / (F) ccount;
+    ccy = ccy / (F) ccount;
+    ax = (x - ccx) * a.collision;
+    ay = (y - ccy) * a.collision;
+  }
+  chx = chx + ax;
+  chy = chy + ay;
+  F nvx = vx + chx * a.global_scale;
+  F nvy = vy + chy * a.global_scale;
+  const F speed = (F) sqrt_rn_f32((float) (nvx * nvx + nvy * nvy));
+  if (speed > a.max_speed) {
+    nvx = (nvx / speed) * a.max_speed;
+    nvy = (nvy / speed) * a.max_speed;
+  }
+  F npx = x + nvx * a.time_scale;
+  F npy = y + nvy * a.time_scale;
+  npx = npx < a.min_pos ? a.max_pos : (npx > a.max_pos ? a.min_pos : npx);
+  npy = npy < a.min_pos ? a.max_pos : (npy > a.max_pos ? a.min_pos : npy);
+  a.opx[i] = npx;
+  a.opy[i] = npy;
+  a.ovx[i] = nvx;
+  a.ovy[i] = nvy;
+}
+
+template <typename F>
+void launch_typed(const StepLaunch &L) {
+  BoidArgs<F> a{};
+  a.px = (const F *) L.in.c[0];
+  a.py = (const F *) L.in.c[1];
+  a.vx = (const F *) L.in.c[2];
+  a.vy = (const F *) L.in.c[3];
+  a.opx = (F *) L.out.c[0];
+  a.opy = (F *) L.out.c[1];
+  a.ovx = (F *) L.out.c[2];
+  a.ovy = (F *) L.out.c[3];
+  a.id = L.nb_id;
+  a.g = L.grid;
+  a.near_t = L.near_t;
+  a.sep_u = sqrtf_lt_threshold((double) (F) L.params[P_SEPARATION_RADIUS]);
+  a.n = L.n;
+  a.time_scale = (F) L.params[P_TIME_SCALE];
+  a.global_scale = (F) L.params[P_GLOBAL_SCALE];
+  a.steer = (F) L.params[P_STEER_SCALE];
+  a.collision = (F) L.params[P_COLLISION_SCALE];
+  a.match = (F) L.params[P_MATCH_SCALE];
+  a.max_speed = (F) L.params[P_MAX_SPEED];
+  a.min_pos = (F) L.params[P_MIN_POS];
+  a.max_pos = (F) L.params[P_MAX_POS];
+  const unsigned nb = grid_blocks(L.n, L.block);
+  if (L.order == 1) {
+    hipLaunchKernelGGL((boids_step<F, true>), nb, L.block, 0, L.stream, a);
+  } else {
+    hipLaunchKernelGGL((boids_step<F, false>), nb, L.block, 0, L.stream, a);
+  }
+}
+
+// main() of boids2d(_cfg).abl before simulate: num_agents times
+//   add(Boid { pos: random(float2(min_pos), float2(max_pos)),
+//              velocity: random(float2(-1), float2(1)) })
+// members initialise in order (pos, then velocity); inside each float2 the
+// y draw comes first (gcc's right-to-left argument order).
+template <typename F>
+int init_typed(abl_sim *sim, const double *p) {
+  const int n = (int) p[P_NUM_AGENTS];
+  const F lo = (F) p[P_MIN_POS], hi = (F) p[P_MAX_POS];
+  std::vector<F> aos(4 * (size_t) (n > 0 ? n : 0));
+  for (int i = 0; i < n; i++) {
+    const F py = host_random<F>(lo, hi);
+    const F px = host_random<F>(lo, hi);
+    const F vy = host_random<F>((F) (-1), (F) 1);
+    const F vx = host_random<F>((F) (-1), (F) 1);
+    F *r = &aos[4 * (size_t) i];
+    r[0] = px;
+    r[1] = py;
+    r[2] = vx;
+    r[3] = vy;
+  }
+  return abl_hip_upload(sim, 0, aos.data(), (size_t) n);
+}
+
+int host_init(abl_sim *sim, const double *p, int precision) {
+  return precision == 0 ? init_typed<double>(sim, p) : init_typed<float>(sim, p);
+}
+
+void launch_update_boid(const StepLaunch &L) {
+  if (L.precision == 0) launch_typed<double>(L);
+  else launch_typed<float>(L);
+}
+
+void env(const double *p, double lo[3], double hi[3], double *g) {
+  lo[0] = lo[1] = p[P_MIN_POS];
+  hi[0] = hi[1] = p[P_MAX_POS];
+  lo[2] = hi[2] = 0.0;
+  *g = p[P_INTERACTION_RADIUS];
+}
+
+}  // namespace
+
+const ProgramDesc *program_boids2d() {
+  static const ProgramDesc d = {
+      "boids2d",
+      {{"Boid", {{"pos", K_FLOAT2, true}, {"velocity", K_FLOAT2, false}}}},
+      {{"interaction_radius", 10.0}, {"separation_radius", 2.0}, {"agent_density", 0.065536},
+       {"time_scale", 0.0005}, {"global_scale", 0.15}, {"steer_scale", 1.0},
+       {"collision_scale", 1.5}, {"match_scale", 1.0}, {"max_speed", 2.0},
+       {"num_agents", 65536}, {"num_timesteps", 200}, {"min_pos", 0.0}, {"max_pos", 1000.0}},
+      {{"update_boid", 0, 0, "interaction_radius", 1.0, 0xfu, launch_update_boid}},
+      env,
+      host_init,
+  };
+  return &d;
+}
+
+}  // namespace abl

@@ -1,0 +1,257 @@
+// prog_circle.hip -- step program for examples/circle.abl (2-D) and
+// circle3d.abl / circle3d_clustered.abl (3-D): the device code `-b hip`
+// emits for step move_point.
+//
+// Arithmetic follows the C backend's generated expression, operation by
+// operation and in the same types (src/backend/CPrinter.cpp:148-173,
+// GenericCPrinter.cpp:20-75, AnalysisVisitor.cpp:89-112, 971-974):
+//   new_pos = in.pos
+//   for nx in near(in, 2.0*r):            dist(nx.pos, in.pos) <= 2.0*r
+//     pos_dist = dist(in.pos, nx.pos)     (abl_float) sqrtf((float) sum of squares)
+//     if (pos_dist == 0) continue
+//     sep_dist = pos_dist - r
+//     force = sep_dist > 0.0
+//       ? ((nx.pos - in.pos) * (abl_float)(k_att * ((2.0*r) - pos_dist))) / pos_dist
+//       : (in.pos - nx.pos) * k_rep
+//     new_pos = new_pos + force
+//   out.pos = clamp(new_pos, float(1), float(W - 1.0))
+// The literals 2.0, 0.0 and 1.0 are C doubles, so under use_float the
+// bracketed scalar expressions are evaluated in double and rounded to float
+// where they are passed on, exactly as the C compiler does.
+// Neighbour state is read through the cell-sorted columns; one thread per
+// agent, the 3x3 (3x3x3) stencil of cells visited row by row.
+#include <math.h>
+
+#include <vector>
+
+#include "abl_runtime.hpp"
+
+namespace abl {
+namespace {
+
+enum { P_NUM_TIMESTEPS, P_NUM_AGENTS, P_RHO, P_K_REP, P_K_ATT, P_R, P_W, P_NUM_CLUSTERS,
+       P_CLUSTER_SIGMA, P_BACKGROUND_FRACTION };
+
+template <typename F, int DIM>
+struct CircleArgs {
+  const F *p[3];   // in.pos columns (cell-sorted)
+  F *o[3];         // out.pos columns
+  const int32_t *id;
+  GridView g;
+  float near_t;
+  int n;
+  F r, k_rep, k_att, W;
+};
+
+template <typename F, int DIM, bool BY_ID>
+__global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  F x[3];
+  F np[3];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    x[d] = a.p[d][i];
+    np[d] = x[d];
+  }
+  const int3 c = cell_of<F>(a.g, x[0], x[1], DIM == 3 ? x[2] : (F) 0);
+  const F r = a.r;
+  auto body = [&](uint32_t j) {
+    F q[3];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) q[d] = a.p[d][j];
+    // near test: dist(nx.pos, in.pos) > 2.0*r -> continue
+    const float f = DIM == 3 ? dist_sq_f32<F>(q[0], q[1], q[2], x[0], x[1], x[2])
+                             : dist_sq_f32<F>(q[0], q[1], x[0], x[1]);
+    if (f > a.near_t) return;
+    // pos_dist = dist(in.pos, nx.pos): same sum of squares (negated terms)
+    const F pos_dist = (F) sqrt_rn_f32(f);
+    if (pos_dist == (F) 0) return;
+    const F sep_dist = pos_dist - r;
+    F force[3];
+    if ((double) sep_dist > 0.0) {
+      const F s = (F) ((double) a.k_att * ((2.0 * (double) r) - (double) pos_dist));
+#pragma unroll
+      for (int d = 0; d < DIM; d++) force[d] = ((q[d] - x[d]) * s) / pos_dist;
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) force[d] = (x[d] - q[d]) * a.k_rep;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) np[d] = np[d] + force[d];
+  };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+  }
+  const F lo = (F) 1;
+  const F hi = (F) ((double) a.W - 1.0);
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const F v = np[d];
+    a.o[d][i] = v < lo ? lo : (v > hi ? hi : v);
+  }
+}
+
+template <typename F, int DIM>
+void launch_typed(const StepLaunch &L) {
+  CircleArgs<F, DIM> a{};
+  for (int d = 0; d < DIM; d++) {
+    a.p[d] = (const F *) L.in.c[d];
+    a.o[d] = (F *) L.out.c[d];
+  }
+  a.id = L.nb_id;
+  a.g = L.grid;
+  a.near_t = L.near_t;
+  a.n = L.n;
+  a.r = (F) L.params[P_R];
+  a.k_rep = (F) L.params[P_K_REP];
+  a.k_att = (F) L.params[P_K_ATT];
+  a.W = (F) L.params[P_W];
+  const unsigned nb = grid_blocks(L.n, L.block);
+  if (L.order == 1) {
+    hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
+  } else {
+    hipLaunchKernelGGL((circle_step<F, DIM, false>), nb, L.block, 0, L.stream, a);
+  }
+}
+
+// main() of circle.abl before simulate: num_agents times
+//   add(Point { pos: random(float2(W)) })
+// random(float2 max) = float2(random(0, max.x), random(0, max.y)) from
+// asset/lib.abl; the C backend's gcc evaluates the two calls right to left,
+// so y takes the first draw and x the second.
+template <typename F>
+int init2(abl_sim *sim, const double *p) {
+  const int n = (int) p[P_NUM_AGENTS];
+  const F W = (F) p[P_W];
+  std::vector<F> aos(2 * (size_t) (n > 0 ? n : 0));
+  for (int i = 0; i < n; i++) {
+    const F y = host_random<F>((F) 0, W);
+    const F x = host_random<F>((F) 0, W);
+    aos[2 * (size_t) i] = x;
+    aos[2 * (size_t) i + 1] = y;
+  }
+  return abl_hip_upload(sim, 0, aos.data(), (size_t) n);
+}
+
+// main() of circle3d_clustered.abl before simulate (num_clusters = 0 gives
+// circle3d.abl's plain random(environment.max)):
+//   per_cluster = num_clusters > 0 ? int((1.0 - bg) * float(N)) / num_clusters : 0
+//   for each cluster: centre = random(env.max);
+//     per_cluster times: dx, dy, dz = gaussian(sigma) (in that order)
+//                        add(Point { pos: clamp(centre + float3(dx,dy,dz), env.max) })
+//   N - per_cluster * num_clusters times: add(Point { pos: random(env.max) })
+// gaussian(s) = s * sqrt(-2 * log(1.0 - u1)) * cos(2.0 * PI * u2), u1 drawn
+// first; PI is the folded literal 3.14159.  float3 random draws run z, y, x
+// (gcc's right-to-left argument order).
+template <typename F>
+F gaussian(F sigma) {
+  const F pi = (F) 3.14159;
+  const F u1 = host_random<F>((F) 0, (F) 1.0);
+  const F u2 = host_random<F>((F) 0, (F) 1.0);
+  return (F) ((sigma * sqrt(((F) (-2)) * log((1.0 - u1)))) * cos(((2.0 * pi) * u2)));
+}
+
+template <typename F>
+void random3(F W, F out[3]) {
+  const F z = host_random<F>((F) 0, W);
+  const F y = host_random<F>((F) 0, W);
+  const F x = host_random<F>((F) 0, W);
+  out[0] = x;
+  out[1] = y;
+  out[2] = z;
+}
+
+template <typename F>
+int init3(abl_sim *sim, const double *p) {
+  const int n = (int) p[P_NUM_AGENTS];
+  const int nclusters = (int) p[P_NUM_CLUSTERS];
+  const F W = (F) p[P_W];
+  const F sigma = (F) p[P_CLUSTER_SIGMA];
+  const F bg = (F) p[P_BACKGROUND_FRACTION];
+  const int per_cluster =
+      nclusters > 0 ? ((int) ((1.0 - bg) * ((F) n))) / nclusters : 0;
+  const int nbackground = n - per_cluster * nclusters;
+  std::vector<F> aos;
+  aos.reserve(3 * (size_t) (n > 0 ? n : 0));
+  for (int c = 0; c < nclusters; c++) {
+    F centre[3];
+    random3<F>(W, centre);
+    for (int i = 0; i < per_cluster; i++) {
+      const F dx = gaussian<F>(sigma);
+      const F dy = gaussian<F>(sigma);
+      const F dz = gaussian<F>(sigma);
+      const F d[3] = {dx, dy, dz};
+      for (int k = 0; k < 3; k++) {
+        const F v = centre[k] + d[k];
+        aos.push_back(v < (F) 0 ? (F) 0 : (v > W ? W : v));
+      }
+    }
+  }
+  for (int i = 0; i < nbackground; i++) {
+    F q[3];
+    random3<F>(W, q);
+    aos.insert(aos.end(), q, q + 3);
+  }
+  return abl_hip_upload(sim, 0, aos.data(), aos.size() / 3);
+}
+
+int host_init2(abl_sim *sim, const double *p, int precision) {
+  return precision == 0 ? init2<double>(sim, p) : init2<float>(sim, p);
+}
+int host_init3(abl_sim *sim, const double *p, int precision) {
+  return precision == 0 ? init3<double>(sim, p) : init3<float>(sim, p);
+}
+
+template <int DIM>
+void launch_move_point(const StepLaunch &L) {
+  if (L.precision == 0) launch_typed<double, DIM>(L);
+  else launch_typed<float, DIM>(L);
+}
+
+void env2(const double *p, double lo[3], double hi[3], double *g) {
+  lo[0] = lo[1] = lo[2] = 0.0;
+  hi[0] = hi[1] = p[P_W];
+  hi[2] = 0.0;
+  *g = 2.0 * p[P_R];
+}
+
+void env3(const double *p, double lo[3], double hi[3], double *g) {
+  lo[0] = lo[1] = lo[2] = 0.0;
+  hi[0] = hi[1] = hi[2] = p[P_W];
+  *g = 2.0 * p[P_R];
+}
+
+}  // namespace
+
+const ProgramDesc *program_circle() {
+  static const ProgramDesc d = {
+      "circle",
+      {{"Point", {{"pos", K_FLOAT2, true}}}},
+      {{"num_timesteps", 100}, {"num_agents", 1000}, {"rho", 0.05}, {"k_rep", 0.05},
+       {"k_att", 0.01}, {"r", 5.0}, {"W", 141.421}},
+      {{"move_point", 0, 0, "r", 2.0, 0x3u, launch_move_point<2>}},
+      env2,
+      host_init2,
+  };
+  return &d;
+}
+
+const ProgramDesc *program_circle3d() {
+  static const ProgramDesc d = {
+      "circle3d",
+      {{"Point", {{"pos", K_FLOAT3, true}}}},
+      {{"num_timesteps", 100}, {"num_agents", 1000}, {"rho", 0.05}, {"k_rep", 0.05},
+       {"k_att", 0.01}, {"r", 5.0}, {"W", 27.1442}, {"num_clusters", 64},
+       {"cluster_sigma", 20.0}, {"background_fraction", 0.2}},
+      {{"move_point", 0, 0, "r", 2.0, 0x7u, launch_move_point<3>}},
+      env3,
+      host_init3,
+  };
+  return &d;
+}
+
+}  // namespace abl
