@@ -122,9 +122,11 @@ int abl_hip_set_param(abl_sim *sim, const char *name, double value);
 int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
 
 /* `-C key=value` runtime options: "hip.block" (threads per block),
- * "hip.cell_scale" (cells per granularity, 1 or 2), "hip.order"
- * ("cell" or "index": visit neighbours in cell order or in the reference's
- * index order). */
+ * "hip.cell_scale" (cells per granularity, 1..4), "hip.order" ("cell" or
+ * "index": visit neighbours in cell order or in the reference's index
+ * order), "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
+ * the counter-based generator random() uses inside steps), "hip.log_csv"
+ * (file log_csv rows are appended to). */
 int abl_hip_set_config(abl_sim *sim, const char *key, const char *value);
 
 /* Agent types of the program, in declaration order. */
@@ -165,6 +167,29 @@ int abl_hip_synchronize(abl_sim *sim);
  * neighbour id.  Pass pairs == NULL to query the count. */
 int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t capacity,
                        size_t *count);
+
+/* Ids of one type's agents, ascending (the order download() returns them).
+ * Ids start as the upload index; after run-time add()/removeCurrent() they
+ * are sparse: survivors keep theirs and newborns take the next ids of their
+ * type in ascending parent id. */
+int abl_hip_download_ids(abl_sim *sim, int agent_type, int64_t *ids, size_t capacity,
+                         size_t *count);
+
+/* ---- sequential-step builtins (src/main.cpp:66-130) -------------------
+ * count(Type) is abl_hip_count.  sum(Type.member) sums per component (bools
+ * count true values) into out[0..components); integer sums are exact and
+ * floating sums are added in a fixed order, so they are deterministic.
+ * count(Type.member, value) counts scalar members equal to value.
+ * getLastExecTime() is the wall time, in seconds, of the last timestep's
+ * parallel steps.  log_csv appends a row (the caller formats it like the
+ * FlameGPU backend, "%d"/"%f" joined by ',', src/backend/FlameGPUPrinter.cpp:
+ * 226-245); rows are kept in memory and appended to the "hip.log_csv" file. */
+int abl_hip_reduce_sum(abl_sim *sim, int agent_type, const char *member, double *out);
+int abl_hip_reduce_count_eq(abl_sim *sim, int agent_type, const char *member, double value,
+                            size_t *count);
+int abl_hip_last_exec_time(abl_sim *sim, double *seconds);
+int abl_hip_log_csv(abl_sim *sim, const char *row);
+int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len);
 
 /* Device memory the sim holds, in bytes. */
 int abl_hip_footprint(abl_sim *sim, size_t *bytes);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -58,7 +59,7 @@ extern "C" const char *abl_hip_last_error(void) { return g_last_error.c_str(); }
 static const ProgramDesc *const *programs() {
   static const ProgramDesc *table[] = {
       program_circle(), program_circle3d(), program_boids2d(), program_game_of_life(),
-      nullptr,
+      program_predator_prey(), nullptr,
   };
   return table;
 }
@@ -104,6 +105,11 @@ struct AgentStore {
   bool grid_valid = false;
   GridView gv{};
   double cell = 0.0;
+  // run-time add/remove: ids stop being 0..n-1 once agents die or are born
+  int64_t next_id = 0;
+  bool ids_dense = true;
+  void *bmem[kMaxCols] = {};  // newborn staging columns, indexed by parent slot
+  size_t bcap = 0;
 };
 
 }  // namespace
@@ -129,6 +135,18 @@ struct abl_sim {
   void *staging = nullptr;
   size_t staging_cap = 0;
   size_t footprint = 0;
+  // add()/removeCurrent() scratch: per-agent flags, their scan, birth list
+  uint8_t *dead = nullptr, *born = nullptr;
+  uint32_t *fpos = nullptr, *fbsum = nullptr, *dscalar = nullptr;
+  int32_t *bl_pid = nullptr;
+  uint32_t *bl_slot = nullptr;
+  size_t fcap = 0;
+  uint64_t seed = 0x5EEDULL;
+  uint64_t timestep = 0;
+  // sequential step: log_csv rows and getLastExecTime
+  std::string log;
+  std::string log_path;
+  double last_exec = 0.0;
 };
 
 // C struct layout of the generated agent struct (src/backend/CPrinter.cpp:250-253):
@@ -171,6 +189,11 @@ static void free_store(AgentStore &s) {
   }
   s.key = s.count = s.start = s.bsum = s.tmp_idx = nullptr;
   s.tmp_id = nullptr;
+  for (int c = 0; c < kMaxCols; c++) {
+    if (s.bmem[c]) (void) hipFree(s.bmem[c]);
+    s.bmem[c] = nullptr;
+  }
+  s.bcap = 0;
   s.n = s.cap = 0;
   s.ncells = s.cells_cap = 0;
   s.grid_valid = false;
@@ -260,12 +283,13 @@ __global__ void k_aos_to_soa(const char *__restrict__ aos, ColXfer x, int32_t *_
   id[i] = i;
 }
 
-// Write slot s back to record id[s] (un-permute to the reference's order).
+// Write slot s back to record id[s] (un-permute to the reference's order),
+// or to record s when id is null.
 __global__ void k_soa_to_aos(char *__restrict__ aos, ColXfer x, const int32_t *__restrict__ id,
                              int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  char *rec = aos + (size_t) id[s] * x.aos_size;
+  char *rec = aos + (size_t) (id ? id[s] : s) * x.aos_size;
   for (int c = 0; c < x.ncols; c++) {
     char *p = rec + x.aos_off[c];
     switch (x.elem[c]) {
@@ -294,7 +318,9 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 // Per tile: exclusive scan of count into start, tile total into bsum.
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *__restrict__ count,
+// INV scans 1 - count (the survivors of a removeCurrent() flag array).
+template <typename T, bool INV>
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const T *__restrict__ count,
                                                           uint32_t *__restrict__ start,
                                                           uint32_t *__restrict__ bsum, int ncells) {
   __shared__ uint32_t lds[kWave + 1];
@@ -303,7 +329,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *__res
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    v[k] = base + k < ncells ? count[base + k] : 0u;
+    uint32_t c = base + k < ncells ? (uint32_t) count[base + k] : (INV ? 1u : 0u);
+    v[k] = INV ? 1u - c : c;
     sum += v[k];
   }
   uint32_t total;
@@ -377,6 +404,94 @@ __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const int32_
       default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
     }
   }
+}
+
+// ---- kernels: run-time removal and birth -----------------------------------------
+
+__global__ void k_zero_u8(uint8_t *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+static __device__ __forceinline__ void copy_cols(const GatherCols &g, uint32_t src, uint32_t dst) {
+  for (int c = 0; c < g.ncols; c++) {
+    switch (g.elem[c]) {
+      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
+      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
+      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
+    }
+  }
+}
+
+// Stable compaction after removeCurrent(): survivor s moves to pos[s], the
+// exclusive prefix count of survivors before it.
+__global__ void k_compact(const uint8_t *__restrict__ dead, const uint32_t *__restrict__ pos,
+                          GatherCols g, const int32_t *__restrict__ id_in,
+                          int32_t *__restrict__ id_out, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || dead[s]) return;
+  const uint32_t d = pos[s];
+  copy_cols(g, (uint32_t) s, d);
+  id_out[d] = id_in[s];
+}
+
+// Births of one step: the k-th parent (in slot order) with born[s] set.
+__global__ void k_birth_list(const uint8_t *__restrict__ born, const uint32_t *__restrict__ pos,
+                             const int32_t *__restrict__ id, int32_t *__restrict__ pid,
+                             uint32_t *__restrict__ slot, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || !born[s]) return;
+  pid[pos[s]] = id[s];
+  slot[pos[s]] = (uint32_t) s;
+}
+
+// Append the newborns after the child type's agents in ascending parent id;
+// they take ids next_id + rank.
+__global__ void k_birth_append(const int32_t *__restrict__ pid, const uint32_t *__restrict__ slot,
+                               int nb, GatherCols g, int32_t *__restrict__ child_id, int n_child,
+                               int32_t next_id) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int32_t my = pid[k];
+  uint32_t rank = 0;
+  for (int q = 0; q < nb; q++) rank += pid[q] < my ? 1u : 0u;
+  const uint32_t dst = (uint32_t) n_child + rank;
+  copy_cols(g, slot[k], dst);
+  child_id[dst] = next_id + (int32_t) rank;
+}
+
+// ---- kernels: reductions for the sequential step -----------------------------------
+
+// Per-block partial sums of a column (int64 for bool/int, double for float);
+// the host adds the partials in block order, so the result is deterministic.
+template <typename T, typename Acc>
+__global__ __launch_bounds__(256) void k_reduce_sum(const T *__restrict__ col, int n,
+                                                    Acc *__restrict__ partial) {
+  __shared__ Acc lds[256];
+  Acc acc = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += (Acc) col[i];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int) threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_count_eq(const T *__restrict__ col, int n, T v,
+                                                         unsigned long long *__restrict__ partial) {
+  __shared__ unsigned long long lds[256];
+  unsigned long long acc = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += col[i] == v ? 1 : 0;
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int) threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
 }
 
 // ---- kernels: neighbour-set probe (parity) -----------------------------------
@@ -477,7 +592,7 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
     }
   }
   const int ntiles = (nc + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_tiles, ntiles, kScanBlock, 0, st, s.count, s.start, s.bsum, nc);
+  hipLaunchKernelGGL((k_scan_tiles<uint32_t, false>), ntiles, kScanBlock, 0, st, s.count, s.start, s.bsum, nc);
   hipLaunchKernelGGL(k_scan_totals, 1, 1024, 0, st, s.bsum, ntiles, s.start, nc);
   hipLaunchKernelGGL(k_scan_add, grid_blocks(nc, 256), 256, 0, st, s.start, s.bsum, nc);
   if (n > 0) {
@@ -586,6 +701,11 @@ extern "C" int abl_hip_destroy(abl_sim *sim) {
   (void) hipStreamSynchronize(sim->stream);
   for (AgentStore &s : sim->types) free_store(s);
   if (sim->staging) (void) hipFree(sim->staging);
+  void *scratch[] = {sim->dead, sim->born, sim->fpos, sim->fbsum, sim->dscalar, sim->bl_pid,
+                     sim->bl_slot};
+  for (void *p : scratch) {
+    if (p) (void) hipFree(p);
+  }
   for (int k = 0; k < 6; k++) {
     if (sim->ev[k]) (void) hipEventDestroy(sim->ev[k]);
   }
@@ -639,6 +759,10 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     else return fail(ABL_ERR_ARG, "hip.order must be cell or index");
   } else if (k == "hip.timing") {
     sim->timing = v == "1" || v == "true";
+  } else if (k == "hip.seed") {
+    sim->seed = std::strtoull(value, nullptr, 0);
+  } else if (k == "hip.log_csv") {
+    sim->log_path = v;
   } else {
     return fail(ABL_ERR_ARG, "unknown config key " + k);
   }
@@ -682,6 +806,8 @@ extern "C" int abl_hip_upload(abl_sim *sim, int t, const void *aos, size_t count
   if (rc) return rc;
   s.n = count;
   s.grid_valid = false;
+  s.ids_dense = true;
+  s.next_id = (int64_t) count;
   if (count == 0) return ABL_OK;
   rc = reserve_staging(sim, count * s.aos_size);
   if (rc) return rc;
@@ -712,11 +838,47 @@ extern "C" int abl_hip_download(abl_sim *sim, int t, void *aos, size_t capacity,
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(sim->staging, 0, s.n * s.aos_size, sim->stream));
   ColXfer x = xfer_of(s, true);
+  if (s.ids_dense) {
+    // ids are 0..n-1: un-permute on the device
+    hipLaunchKernelGGL(k_soa_to_aos, grid_blocks((int) s.n, 256), 256, 0, sim->stream,
+                       (char *) sim->staging, x, s.id[s.id_cur], (int) s.n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(aos, sim->staging, s.n * s.aos_size, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    return ABL_OK;
+  }
+  // after add/remove: records in slot order, then ordered by id on the host
   hipLaunchKernelGGL(k_soa_to_aos, grid_blocks((int) s.n, 256), 256, 0, sim->stream,
-                     (char *) sim->staging, x, s.id[s.id_cur], (int) s.n);
+                     (char *) sim->staging, x, (const int32_t *) nullptr, (int) s.n);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(aos, sim->staging, s.n * s.aos_size, hipMemcpyDeviceToHost, sim->stream));
+  std::vector<char> recs(s.n * s.aos_size);
+  std::vector<int32_t> ids(s.n);
+  HIP_TRY(hipMemcpyAsync(recs.data(), sim->staging, recs.size(), hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(hipMemcpyAsync(ids.data(), s.id[s.id_cur], s.n * 4, hipMemcpyDeviceToHost, sim->stream));
   HIP_TRY(hipStreamSynchronize(sim->stream));
+  std::vector<uint32_t> order(s.n);
+  for (size_t i = 0; i < s.n; i++) order[i] = (uint32_t) i;
+  std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) { return ids[p] < ids[q]; });
+  for (size_t i = 0; i < s.n; i++) {
+    std::memcpy((char *) aos + i * s.aos_size, recs.data() + (size_t) order[i] * s.aos_size, s.aos_size);
+  }
+  return ABL_OK;
+}
+
+// Ids of the agents of one type in download order (ascending).
+extern "C" int abl_hip_download_ids(abl_sim *sim, int t, int64_t *ids, size_t capacity,
+                                    size_t *count) {
+  CHECK_SIM(sim);
+  if (t < 0 || t >= (int) sim->types.size()) return fail(ABL_ERR_ARG, "agent type out of range");
+  AgentStore &s = sim->types[t];
+  if (count) *count = s.n;
+  if (s.n == 0) return ABL_OK;
+  if (!ids || capacity < s.n) return fail(ABL_ERR_ARG, "id buffer too small");
+  std::vector<int32_t> h(s.n);
+  HIP_TRY(hipMemcpyAsync(h.data(), s.id[s.id_cur], s.n * 4, hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  std::sort(h.begin(), h.end());
+  for (size_t i = 0; i < s.n; i++) ids[i] = h[i];
   return ABL_OK;
 }
 
@@ -770,8 +932,118 @@ extern "C" int abl_hip_init_agents(abl_sim *sim) {
   return sim->prog->host_init(sim, sim->params.data(), sim->precision);
 }
 
-static int launch_step(abl_sim *sim, const StepDesc &st) {
+// ---- run-time add()/removeCurrent() ------------------------------------------------
+
+static int reserve_flags(abl_sim *sim, size_t n) {
+  if (n <= sim->fcap && sim->dead) return ABL_OK;
+  size_t cap = std::max<size_t>(n, 1024);
+  void *ptrs[] = {sim->dead, sim->born, sim->fpos, sim->fbsum, sim->bl_pid, sim->bl_slot};
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  for (void *p : ptrs) {
+    if (p) HIP_TRY(hipFree(p));
+  }
+  HIP_TRY(hipMalloc(&sim->dead, cap));
+  HIP_TRY(hipMalloc(&sim->born, cap));
+  HIP_TRY(hipMalloc(&sim->fpos, (cap + 1) * 4));
+  HIP_TRY(hipMalloc(&sim->fbsum, ((cap + kScanTile - 1) / kScanTile + 1) * 4));
+  HIP_TRY(hipMalloc(&sim->bl_pid, cap * 4));
+  HIP_TRY(hipMalloc(&sim->bl_slot, cap * 4));
+  sim->fcap = cap;
+  return ABL_OK;
+}
+
+// Exclusive scan of n flags into sim->fpos (INV: of 1 - flag); returns the
+// total through *total (one read-back).
+template <bool INV>
+static int scan_flags(abl_sim *sim, const uint8_t *flags, int n, uint32_t *total) {
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  hipStream_t st = sim->stream;
+  hipLaunchKernelGGL((k_scan_tiles<uint8_t, INV>), ntiles, kScanBlock, 0, st, flags, sim->fpos,
+                     sim->fbsum, n);
+  hipLaunchKernelGGL(k_scan_totals, 1, 1024, 0, st, sim->fbsum, ntiles, sim->fpos, n);
+  hipLaunchKernelGGL(k_scan_add, grid_blocks(n, 256), 256, 0, st, sim->fpos, sim->fbsum, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(total, sim->fpos + n, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return ABL_OK;
+}
+
+static int reserve_births(abl_sim *sim, AgentStore &child, size_t parents) {
+  if (parents <= child.bcap && child.bmem[0]) return ABL_OK;
+  size_t cap = std::max<size_t>(parents, 1024);
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  for (size_t c = 0; c < child.cols.size(); c++) {
+    if (child.bmem[c]) HIP_TRY(hipFree(child.bmem[c]));
+    child.bmem[c] = nullptr;
+    HIP_TRY(hipMalloc(&child.bmem[c], cap * (size_t) child.cols[c].elem));
+  }
+  child.bcap = cap;
+  return ABL_OK;
+}
+
+// Drop the agents whose removeCurrent() flag is set, keeping order.
+static int apply_removals(abl_sim *sim, AgentStore &a, int n) {
+  uint32_t alive = 0;
+  int rc = scan_flags<true>(sim, sim->dead, n, &alive);
+  if (rc) return rc;
+  if ((int) alive == n) return ABL_OK;
+  GatherCols g{};
+  g.ncols = (int) a.cols.size();
+  for (int c = 0; c < g.ncols; c++) {
+    g.src[c] = col_cur(a, c);
+    g.dst[c] = col_other(a, c);
+    g.elem[c] = a.cols[c].elem;
+  }
+  hipLaunchKernelGGL(k_compact, grid_blocks(n, 256), 256, 0, sim->stream, sim->dead, sim->fpos, g,
+                     a.id[a.id_cur], a.id[1 - a.id_cur], n);
+  HIP_TRY(hipGetLastError());
+  for (int c = 0; c < g.ncols; c++) a.cur[c] ^= 1;
+  a.id_cur ^= 1;
+  a.n = alive;
+  a.ids_dense = false;
+  a.grid_valid = false;
+  return ABL_OK;
+}
+
+// Append the step's newborns (staged per parent slot) to the child type.
+static int apply_births(abl_sim *sim, AgentStore &parent, AgentStore &child, int n_parent) {
+  uint32_t nb = 0;
+  int rc = scan_flags<false>(sim, sim->born, n_parent, &nb);
+  if (rc) return rc;
+  if (nb == 0) return ABL_OK;
+  hipLaunchKernelGGL(k_birth_list, grid_blocks(n_parent, 256), 256, 0, sim->stream, sim->born,
+                     sim->fpos, parent.id[parent.id_cur], sim->bl_pid, sim->bl_slot, n_parent);
+  GatherCols g{};
+  g.ncols = (int) child.cols.size();
+  for (int c = 0; c < g.ncols; c++) {
+    g.src[c] = child.bmem[c];
+    g.dst[c] = col_cur(child, c);
+    g.elem[c] = child.cols[c].elem;
+  }
+  hipLaunchKernelGGL(k_birth_append, grid_blocks((int) nb, 256), 256, 0, sim->stream, sim->bl_pid,
+                     sim->bl_slot, (int) nb, g, child.id[child.id_cur], (int) child.n,
+                     (int32_t) child.next_id);
+  HIP_TRY(hipGetLastError());
+  child.n += nb;
+  child.next_id += nb;
+  child.ids_dense = false;
+  child.grid_valid = false;
+  return ABL_OK;
+}
+
+static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
   AgentStore &a = sim->types[st.agent];
+  if (st.removes || st.adds) {
+    int rc = reserve_flags(sim, 2 * a.n);
+    if (rc) return rc;
+    if (st.adds) {
+      AgentStore &c = sim->types[st.adds - 1];
+      rc = reserve_store(sim, c, c.n + a.n);
+      if (rc) return rc;
+      rc = reserve_births(sim, c, a.n);
+      if (rc) return rc;
+    }
+  }
   StepLaunch L{};
   L.stream = sim->stream;
   L.precision = sim->precision;
@@ -810,6 +1082,21 @@ static int launch_step(abl_sim *sim, const StepDesc &st) {
       if (runs > 27) return fail(ABL_ERR_UNSUPPORTED, "hip.order=index needs a stencil of at most 27 cells");
     }
   }
+  L.self_id = a.id[a.id_cur];
+  L.seed = sim->seed;
+  L.timestep = sim->timestep;
+  L.step_ordinal = ordinal;
+  const int n_step = L.n;
+  if (n_step > 0 && (st.removes || st.adds)) {
+    L.dead = sim->dead;
+    L.born = sim->born;
+    hipLaunchKernelGGL(k_zero_u8, grid_blocks(n_step, 256), 256, 0, sim->stream, sim->dead, n_step);
+    hipLaunchKernelGGL(k_zero_u8, grid_blocks(n_step, 256), 256, 0, sim->stream, sim->born, n_step);
+    if (st.adds) {
+      AgentStore &c = sim->types[st.adds - 1];
+      for (size_t k = 0; k < c.cols.size(); k++) L.birth.c[k] = c.bmem[k];
+    }
+  }
   if (L.n > 0) {
     int sp = span_open(sim, 1);
     st.launch(L);
@@ -825,6 +1112,26 @@ static int launch_step(abl_sim *sim, const StepDesc &st) {
       if ((w >> (a.pos_col + d)) & 1u) a.grid_valid = false;
     }
   }
+  // Births first (their flags are indexed by the pre-step slots), then the
+  // order-keeping removal; newborns of the same type sit after the parents
+  // and are never removed, so the result is survivors in order followed by
+  // newborns in parent-id order.
+  int extra = 0;
+  if (n_step > 0 && st.adds) {
+    AgentStore &c = sim->types[st.adds - 1];
+    const size_t before = c.n;
+    int rc = apply_births(sim, a, c, n_step);
+    if (rc) return rc;
+    if (&c == &a) extra = (int) (c.n - before);
+  }
+  if (n_step > 0 && st.removes) {
+    if (extra > 0) {
+      hipLaunchKernelGGL(k_zero_u8, grid_blocks(extra, 256), 256, 0, sim->stream,
+                         sim->dead + n_step, extra);
+    }
+    int rc = apply_removals(sim, a, n_step + extra);
+    if (rc) return rc;
+  }
   return ABL_OK;
 }
 
@@ -834,11 +1141,23 @@ extern "C" int abl_hip_simulate(abl_sim *sim, int timesteps) {
   sim->evused = 0;
   sim->spans.clear();
   if (sim->timing) HIP_TRY(hipEventRecord(sim->ev[0], sim->stream));
+  auto t_prev = std::chrono::steady_clock::now();
   for (int t = 0; t < timesteps; t++) {
+    uint32_t ordinal = 0;
     for (const StepDesc &st : sim->prog->steps) {
-      int rc = launch_step(sim, st);
+      int rc = launch_step(sim, st, ordinal++);
       if (rc) return rc;
     }
+    if (sim->prog->seq_step) {
+      // getLastExecTime(): seconds taken by this timestep's parallel steps
+      HIP_TRY(hipStreamSynchronize(sim->stream));
+      auto now = std::chrono::steady_clock::now();
+      sim->last_exec = std::chrono::duration<double>(now - t_prev).count();
+      int rc = sim->prog->seq_step(sim, sim->params.data());
+      if (rc) return rc;
+      t_prev = std::chrono::steady_clock::now();
+    }
+    sim->timestep++;
   }
   if (sim->timing) {
     HIP_TRY(hipEventRecord(sim->ev[1], sim->stream));
@@ -950,5 +1269,144 @@ extern "C" int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipFree(cnt));
   HIP_TRY(hipFree(off));
+  return ABL_OK;
+}
+
+// ---- sequential-step builtins (src/main.cpp:66-130) -------------------------------
+
+static int member_columns(abl_sim *sim, int t, const char *member, int *first, int *ncomp) {
+  if (t < 0 || t >= (int) sim->types.size() || !member) return fail(ABL_ERR_ARG, "bad argument");
+  const AgentStore &s = sim->types[t];
+  for (size_t c = 0; c < s.cols.size(); c++) {
+    if (std::strcmp(s.desc->members[s.cols[c].member].name, member) == 0) {
+      *first = (int) c;
+      *ncomp = kind_components(s.desc->members[s.cols[c].member].kind);
+      return ABL_OK;
+    }
+  }
+  return fail(ABL_ERR_ARG, std::string("Agent has no member \"") + member + "\"");
+}
+
+// sum(Type.member): bool sums count the true values; vectors sum per component.
+extern "C" int abl_hip_reduce_sum(abl_sim *sim, int t, const char *member, double *out) {
+  CHECK_SIM(sim);
+  int c0 = 0, nc = 0;
+  int rc = member_columns(sim, t, member, &c0, &nc);
+  if (rc) return rc;
+  if (!out) return fail(ABL_ERR_ARG, "null output");
+  AgentStore &s = sim->types[t];
+  const int n = (int) s.n;
+  const int blocks = std::max(1, std::min(1024, (n + 255) / 256));
+  const int kind = s.desc->members[s.cols[c0].member].kind;
+  void *partial = nullptr;
+  HIP_TRY(hipMalloc(&partial, (size_t) blocks * 8));
+  for (int k = 0; k < nc; k++) {
+    const void *col = col_cur(s, c0 + k);
+    if (n == 0) {
+      out[k] = 0.0;
+      continue;
+    }
+    if (kind == K_BOOL) {
+      hipLaunchKernelGGL((k_reduce_sum<uint8_t, long long>), blocks, 256, 0, sim->stream,
+                         (const uint8_t *) col, n, (long long *) partial);
+    } else if (kind == K_INT) {
+      hipLaunchKernelGGL((k_reduce_sum<int32_t, long long>), blocks, 256, 0, sim->stream,
+                         (const int32_t *) col, n, (long long *) partial);
+    } else if (s.cols[c0].elem == 8) {
+      hipLaunchKernelGGL((k_reduce_sum<double, double>), blocks, 256, 0, sim->stream,
+                         (const double *) col, n, (double *) partial);
+    } else {
+      hipLaunchKernelGGL((k_reduce_sum<float, double>), blocks, 256, 0, sim->stream,
+                         (const float *) col, n, (double *) partial);
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<char> h((size_t) blocks * 8);
+    HIP_TRY(hipMemcpyAsync(h.data(), partial, h.size(), hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    double acc = 0.0;
+    long long iacc = 0;
+    for (int b = 0; b < blocks; b++) {
+      if (kind == K_BOOL || kind == K_INT) iacc += ((long long *) h.data())[b];
+      else acc += ((double *) h.data())[b];
+    }
+    out[k] = (kind == K_BOOL || kind == K_INT) ? (double) iacc : acc;
+  }
+  HIP_TRY(hipFree(partial));
+  return ABL_OK;
+}
+
+// count(Type.member, value): agents whose (scalar) member equals value.
+extern "C" int abl_hip_reduce_count_eq(abl_sim *sim, int t, const char *member, double value,
+                                       size_t *out) {
+  CHECK_SIM(sim);
+  int c0 = 0, nc = 0;
+  int rc = member_columns(sim, t, member, &c0, &nc);
+  if (rc) return rc;
+  if (!out) return fail(ABL_ERR_ARG, "null output");
+  if (nc != 1) return fail(ABL_ERR_UNSUPPORTED, "count(member, value) on a vector member");
+  AgentStore &s = sim->types[t];
+  const int n = (int) s.n;
+  *out = 0;
+  if (n == 0) return ABL_OK;
+  const int blocks = std::max(1, std::min(1024, (n + 255) / 256));
+  const int kind = s.desc->members[s.cols[c0].member].kind;
+  unsigned long long *partial = nullptr;
+  HIP_TRY(hipMalloc(&partial, (size_t) blocks * 8));
+  const void *col = col_cur(s, c0);
+  if (kind == K_BOOL) {
+    hipLaunchKernelGGL(k_reduce_count_eq<uint8_t>, blocks, 256, 0, sim->stream,
+                       (const uint8_t *) col, n, (uint8_t) (value != 0.0), partial);
+  } else if (kind == K_INT) {
+    hipLaunchKernelGGL(k_reduce_count_eq<int32_t>, blocks, 256, 0, sim->stream,
+                       (const int32_t *) col, n, (int32_t) value, partial);
+  } else if (s.cols[c0].elem == 8) {
+    hipLaunchKernelGGL(k_reduce_count_eq<double>, blocks, 256, 0, sim->stream,
+                       (const double *) col, n, value, partial);
+  } else {
+    hipLaunchKernelGGL(k_reduce_count_eq<float>, blocks, 256, 0, sim->stream,
+                       (const float *) col, n, (float) value, partial);
+  }
+  HIP_TRY(hipGetLastError());
+  std::vector<unsigned long long> h((size_t) blocks);
+  HIP_TRY(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, sim->stream));
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  HIP_TRY(hipFree(partial));
+  unsigned long long acc = 0;
+  for (unsigned long long v : h) acc += v;
+  *out = (size_t) acc;
+  return ABL_OK;
+}
+
+// getLastExecTime(): wall-clock seconds of the last timestep's parallel steps.
+extern "C" int abl_hip_last_exec_time(abl_sim *sim, double *seconds) {
+  CHECK_SIM(sim);
+  if (!seconds) return fail(ABL_ERR_ARG, "null output");
+  *seconds = sim->last_exec;
+  return ABL_OK;
+}
+
+// log_csv(...): append one row; the runtime keeps the text and, when
+// "hip.log_csv" names a file, appends it there too.
+extern "C" int abl_hip_log_csv(abl_sim *sim, const char *row) {
+  CHECK_SIM(sim);
+  if (!row) return fail(ABL_ERR_ARG, "null row");
+  sim->log += row;
+  if (!sim->log_path.empty()) {
+    FILE *f = std::fopen(sim->log_path.c_str(), "a");
+    if (!f) return fail(ABL_ERR_ARG, "cannot open " + sim->log_path);
+    std::fputs(row, f);
+    std::fclose(f);
+  }
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len) {
+  CHECK_SIM(sim);
+  if (len) *len = sim->log.size();
+  if (buf && capacity) {
+    size_t m = std::min(capacity - 1, sim->log.size());
+    std::memcpy(buf, sim->log.data(), m);
+    buf[m] = 0;
+  }
   return ABL_OK;
 }

@@ -52,6 +52,8 @@ struct StepDesc {
   double radius_scale;
   uint32_t writes;           // bit per column index of `agent` the step writes
   void (*launch)(const StepLaunch &);
+  bool removes;              // calls removeCurrent()
+  int adds;                  // 0, or 1 + the agent type it add()s at run time
 };
 
 struct ProgramDesc {
@@ -65,6 +67,9 @@ struct ProgramDesc {
   // main() statements before simulate(...): builds every agent type's
   // records on the host with the runtime's generator and uploads them.
   int (*host_init)(struct ::abl_sim *sim, const double *params, int precision);
+  // sequential step (run on the host once per timestep after the parallel
+  // steps; uses the reduction entry points), or nullptr
+  int (*seq_step)(struct ::abl_sim *sim, const double *params);
 };
 
 // Column pointers of one agent type (members in declaration order, vector
@@ -87,7 +92,37 @@ struct StepLaunch {
   const double *params;  // host copy, indexed as ProgramDesc::params
   int block;           // threads per block
   int order;           // 0 = cell order, 1 = reference index order
+  // run-time add()/removeCurrent() and random() inside steps
+  const int32_t *self_id;  // ids of the stepping agents
+  uint8_t *dead;           // removeCurrent() flags, per stepping agent
+  uint8_t *born;           // add() flags, per stepping agent
+  ColSet birth;            // newborn columns (the added type), per parent slot
+  uint64_t seed;
+  uint64_t timestep;
+  uint32_t step_ordinal;
 };
+
+// random() inside a step: a counter-based draw keyed by (seed, timestep,
+// step ordinal, agent id, call index), so results do not depend on which
+// thread runs which agent or on the order agents are stored in.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t counter_bits(uint64_t seed, uint64_t t, uint32_t step,
+                                                 uint64_t id, uint32_t k) {
+  uint64_t z = mix64(seed + 0x9E3779B97F4A7C15ULL * (t + 1));
+  z = mix64(z ^ (((uint64_t) step << 32) | k));
+  return mix64(z + id);
+}
+// random_float(lo, hi) on those bits, in the build's float type
+// (asset/c/libabl.c:19-24 formula).
+template <typename F>
+__host__ __device__ inline F counter_random(F lo, F hi, uint64_t bits) {
+  const F span = (F) 18446744073709551615.0 / (hi - lo);
+  return lo + (F) bits / span;
+}
 
 // Host-side helpers (abl_host.cpp).
 float sqrtf_le_threshold(double r);  // max float f >= 0 with sqrtf(f) <= r
@@ -98,6 +133,7 @@ const ProgramDesc *program_circle();
 const ProgramDesc *program_circle3d();
 const ProgramDesc *program_boids2d();
 const ProgramDesc *program_game_of_life();
+const ProgramDesc *program_predator_prey();
 
 inline double param_of(const StepLaunch &L, int idx) { return L.params[idx]; }
 

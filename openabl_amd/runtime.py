@@ -57,6 +57,12 @@ _SIGNATURES = {
     "abl_hip_neighbours": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p], C.c_int),
     "abl_hip_footprint": ([C.c_void_p, C.c_void_p], C.c_int),
     "abl_hip_last_timing": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_hip_download_ids": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_hip_reduce_sum": ([C.c_void_p, C.c_int, C.c_char_p, C.c_void_p], C.c_int),
+    "abl_hip_reduce_count_eq": ([C.c_void_p, C.c_int, C.c_char_p, C.c_double, C.c_void_p], C.c_int),
+    "abl_hip_last_exec_time": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_hip_log_csv": ([C.c_void_p, C.c_char_p], C.c_int),
+    "abl_hip_log_text": ([C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p], C.c_int),
 }
 
 
@@ -113,6 +119,16 @@ def derived_params(program: str, params: dict) -> dict:
     elif program == "game_of_life":
         if "num_agents" in p:
             p["size"] = int(math.sqrt(p["num_agents"]))
+    elif program == "predator_prey":
+        sm = p.get("SPACE_MULT", 1.0)
+        if "SPACE_MULT" in p:
+            for name, base in (("PRED_PREY_INTERACTION_RADIUS", 0.1),
+                               ("PREY_GROUP_COHESION_RADIUS", 0.12),
+                               ("SAME_SPECIES_AVOIDANCE_RADIUS", 0.035),
+                               ("GRASS_EAT_DISTANCE", 0.02), ("PRED_KILL_DISTANCE", 0.02)):
+                p[name] = fold6(base * sm)
+        if "num_agents" in p or "agent_density" in p or "SPACE_MULT" in p:
+            p["env_size"] = fold6(math.sqrt(p.get("num_agents", 3200) / p.get("agent_density", 1600.0)) * sm)
     return p
 
 
@@ -212,6 +228,31 @@ class Simulation:
         pairs = out[: 2 * got.value].reshape(-1, 2)
         return pairs[:, 0].copy(), pairs[:, 1].copy()
 
+    def ids(self, t: int = 0) -> np.ndarray:
+        """Ids of type t's agents in download order."""
+        n = self.count(t)
+        out = np.zeros(max(n, 1), dtype=np.int64)
+        got = C.c_size_t()
+        _check(self.lib.abl_hip_download_ids(self.h, t, out.ctypes.data, n, C.byref(got)))
+        return out[: got.value]
+
+    def reduce_sum(self, t: int, member: str, ncomp: int = 1):
+        out = (C.c_double * ncomp)()
+        _check(self.lib.abl_hip_reduce_sum(self.h, t, member.encode(), out))
+        return list(out) if ncomp > 1 else out[0]
+
+    def count_eq(self, t: int, member: str, value) -> int:
+        n = C.c_size_t()
+        _check(self.lib.abl_hip_reduce_count_eq(self.h, t, member.encode(), float(value), C.byref(n)))
+        return n.value
+
+    def log_text(self) -> str:
+        n = C.c_size_t()
+        _check(self.lib.abl_hip_log_text(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(self.lib.abl_hip_log_text(self.h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
     def footprint(self) -> int:
         b = C.c_size_t()
         _check(self.lib.abl_hip_footprint(self.h, C.byref(b)))
@@ -245,10 +286,16 @@ def record_dtype(members, precision: str = "f64") -> np.dtype:
     return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": itemsize})
 
 
-MEMBERS = {
-    "circle": [("pos", "float2")],
-    "circle3d": [("pos", "float3")],
-    "boids2d": [("pos", "float2"), ("velocity", "float2")],
-    "game_of_life": [("pos", "float2"), ("alive", "bool")],
+_MOVER = [("pos", "float2"), ("dir", "float2"), ("steer", "float2"), ("life", "int")]
+
+# Agent types of each program, in declaration order, with their members.
+AGENTS = {
+    "circle": [("Point", [("pos", "float2")])],
+    "circle3d": [("Point", [("pos", "float3")])],
+    "boids2d": [("Boid", [("pos", "float2"), ("velocity", "float2")])],
+    "game_of_life": [("Cell", [("pos", "float2"), ("alive", "bool")])],
+    "predator_prey": [("Predator", _MOVER), ("Prey", _MOVER),
+                      ("Grass", [("pos", "float2"), ("dead_cycles", "int"), ("avail", "bool")])],
 }
-AGENT_NAMES = {"circle": "Point", "circle3d": "Point", "boids2d": "Boid", "game_of_life": "Cell"}
+MEMBERS = {k: v[0][1] for k, v in AGENTS.items()}
+AGENT_NAMES = {k: v[0][0] for k, v in AGENTS.items()}

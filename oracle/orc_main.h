@@ -20,6 +20,7 @@
 int orc_set_param(const char *name, double value);
 double orc_get_param(const char *name);
 void orc_simulate(int timesteps);
+void orc_init(void);
 void orc_save(const char *path);
 size_t orc_agent_count(int t);
 int orc_num_agent_types(void);
@@ -47,7 +48,7 @@ void orc_sample_step(int s, size_t m);
       }                                                                        \
     }                                                                          \
     if (steps < 0) steps = (int) orc_get_param("num_timesteps");              \
-    orc_model.init();                                                          \
+    orc_init();                                                                \
     if (sample > 0) {                                                          \
       struct timespec s0, s1;                                                  \
       clock_gettime(CLOCK_MONOTONIC, &s0);                                     \

@@ -22,6 +22,13 @@ extern void orc_record_reset(void);
 extern size_t orc_record_count(void);
 extern const long *orc_record_data(void);
 
+#ifndef ORC_REF_BUILD
+void orc_dyn_begin_step(const orc_step *st, size_t n);
+void orc_dyn_set_agent(int type, size_t i);
+void orc_dyn_end_step(const orc_step *st, size_t n);
+void orc_ids_reset(int t, size_t n);
+#endif
+
 static const orc_param *find_param(const char *name) {
   for (const orc_param *p = orc_model.params; p->name; p++) {
     if (strcmp(p->name, name) == 0) return p;
@@ -87,9 +94,14 @@ void orc_reset(void) {
 void orc_init(void) {
   orc_reset();
   orc_model.init();
+#ifndef ORC_REF_BUILD
+  for (int t = 0; t < orc_model.num_agents; t++) orc_ids_reset(t, orc_model.agents[t].buf->len);
+  orc_timestep = 0;
+#endif
 }
 
-/* Replace an agent array with `n` records in the generated struct layout. */
+/* Replace an agent array with `n` records in the generated struct layout;
+ * they take ids 0..n-1. */
 void orc_agent_set(int t, const void *data, size_t n) {
   const orc_agent *a = &orc_model.agents[t];
   free_array(a->buf);
@@ -97,6 +109,9 @@ void orc_agent_set(int t, const void *data, size_t n) {
   for (size_t i = 0; i < n; i++) {
     memcpy(dyn_array_place(a->buf, a->size), (const char *) data + i * a->size, a->size);
   }
+#ifndef ORC_REF_BUILD
+  orc_ids_reset(t, n);
+#endif
 }
 
 static void build_grid(const orc_step *st) {
@@ -110,7 +125,14 @@ static void build_grid(const orc_step *st) {
 static void run_step(int s) {
   const orc_step *st = &orc_model.steps[s];
   const orc_agent *a = &orc_model.agents[st->agent];
-  if (!a->dbuf->values) {
+  if (orc_model.dynamic) {
+#ifndef ORC_REF_BUILD
+    /* copy-through: out starts as in (sizes may have changed) */
+    free(a->dbuf->values);
+    *a->dbuf = dyn_array_copy_fixed(a->size, a->buf);
+    orc_step_ordinal = s;
+#endif
+  } else if (!a->dbuf->values) {
     *a->dbuf = dyn_array_copy_fixed(a->size, a->buf);
   }
 #ifdef ORACLE_GRID
@@ -121,18 +143,39 @@ static void run_step(int s) {
   size_t n = a->buf->len;
   size_t sz = a->size;
   void (*fn)(void *, void *) = st->fn;
+  if (orc_model.dynamic) {
+#ifndef ORC_REF_BUILD
+    orc_dyn_begin_step(st, n);
+    const int type = st->agent;
 #pragma omp parallel for
-  for (size_t i = 0; i < n; i++) {
-    fn(in + i * sz, out + i * sz);
+    for (size_t i = 0; i < n; i++) {
+      orc_dyn_set_agent(type, i);
+      fn(in + i * sz, out + i * sz);
+    }
+#endif
+  } else {
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++) {
+      fn(in + i * sz, out + i * sz);
+    }
   }
   dyn_array tmp = *a->buf;
   *a->buf = *a->dbuf;
   *a->dbuf = tmp;
+#ifndef ORC_REF_BUILD
+  if (orc_model.dynamic) orc_dyn_end_step(st, n);
+#endif
 }
 
 void orc_simulate(int timesteps) {
   for (int t = 0; t < timesteps; t++) {
     for (int s = 0; s < orc_model.num_steps; s++) run_step(s);
+#ifndef ORC_REF_BUILD
+    if (orc_model.dynamic) {
+      if (orc_model.seq_step) orc_model.seq_step();
+      orc_timestep++;
+    }
+#endif
   }
 }
 

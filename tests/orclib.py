@@ -177,6 +177,22 @@ class Oracle:
     def save(self, path: str):
         self.lib.orc_save(path.encode())
 
+    # -b hip semantics of dynamic models (oracle/orc_dynamic.c) ------------
+    def ids(self, t: int = 0) -> np.ndarray:
+        self.lib.orc_agent_ids.restype = C.POINTER(C.c_longlong)
+        n = self.lib.orc_agent_count(t)
+        if n == 0:
+            return np.zeros(0, dtype=np.int64)
+        return np.ctypeslib.as_array(self.lib.orc_agent_ids(t), shape=(n,)).astype(np.int64)
+
+    def log_text(self) -> str:
+        self.lib.orc_log_text.restype = C.c_char_p
+        return self.lib.orc_log_text().decode()
+
+    def set_seed(self, seed: int):
+        self.lib.orc_set_seed.argtypes = [C.c_ulonglong]
+        self.lib.orc_set_seed(seed)
+
 
 def ref_binary(model: str, precision: str = "f64") -> str | None:
     """Path of the reference-runtime build of a restated model, if built."""
@@ -209,6 +225,21 @@ def boids2d_params(num_agents=65536, agent_density=0.065536, interaction_radius=
                 match_scale=match_scale, max_speed=max_speed, time_scale=time_scale,
                 global_scale=global_scale, num_timesteps=num_timesteps,
                 min_pos=0.0, max_pos=fold6(math.sqrt(num_agents / agent_density)))
+
+
+def predator_prey_params(num_agents=3200, prey_fraction=0.35, predator_fraction=0.025,
+                         agent_density=1600.0, SPACE_MULT=1.0, num_timesteps=100, **extra):
+    p = dict(num_agents=num_agents, prey_fraction=prey_fraction,
+             predator_fraction=predator_fraction, agent_density=agent_density,
+             SPACE_MULT=SPACE_MULT, num_timesteps=num_timesteps,
+             PRED_PREY_INTERACTION_RADIUS=fold6(0.1 * SPACE_MULT),
+             PREY_GROUP_COHESION_RADIUS=fold6(0.12 * SPACE_MULT),
+             SAME_SPECIES_AVOIDANCE_RADIUS=fold6(0.035 * SPACE_MULT),
+             GRASS_EAT_DISTANCE=fold6(0.02 * SPACE_MULT),
+             PRED_KILL_DISTANCE=fold6(0.02 * SPACE_MULT),
+             env_size=fold6(math.sqrt(num_agents / agent_density) * SPACE_MULT))
+    p.update(extra)
+    return p
 
 
 def game_of_life_params(num_agents=400, alive_fraction=0.1, debug=False, num_timesteps=100):
