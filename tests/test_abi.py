@@ -46,7 +46,7 @@ def test_python_binding_covers_header(hip):
 
 def test_programs_listed(hip):
     names = [hip.abl_hip_program_name(i).decode() for i in range(hip.abl_hip_program_count())]
-    assert names == ["circle", "circle3d", "boids2d", "game_of_life"]
+    assert names == ["circle", "circle3d", "boids2d", "game_of_life", "predator_prey"]
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
