@@ -124,6 +124,7 @@ struct abl_sim {
   int block = 256;
   int cell_scale = 1;
   int order = 0;
+  int tile = 1;
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -759,6 +760,10 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     else return fail(ABL_ERR_ARG, "hip.order must be cell or index");
   } else if (k == "hip.timing") {
     sim->timing = v == "1" || v == "true";
+  } else if (k == "hip.tile") {
+    int t = std::atoi(value);
+    if (t < 0 || t > 3) return fail(ABL_ERR_ARG, "hip.tile must be 0..3");
+    sim->tile = t;
   } else if (k == "hip.seed") {
     sim->seed = std::strtoull(value, nullptr, 0);
   } else if (k == "hip.log_csv") {
@@ -1076,6 +1081,9 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
     L.grid = b.gv;
     L.grid.reach = reach_for(b, r);
     L.near_t = sqrtf_le_threshold(r);
+    L.tile = sim->tile;
+    L.occupancy = b.ncells ? (float) ((double) b.n / (double) b.ncells) : 0.f;
+    L.rc = (float) (r / b.cell * (1.0 + 1.0 / 1048576.0));
     if (sim->order == 1) {
       int side = 2 * L.grid.reach + 1;
       int runs = side * side * (b.dim == 3 ? side : 1);

@@ -1,0 +1,165 @@
+// abl_tile.hpp -- LDS-tiled neighbour iteration for the step programs.
+//
+// One block owns a tile of TX x TY (x TZ) grid cells.  It stages, for the
+// tile plus a halo of `reach` cells on every side, the neighbour type's
+// positions and the other members the step reads, into LDS as one record
+// (array of structs) per agent: every halo row of cells is one contiguous
+// range of the cell-sorted columns, so the staging loads are coalesced.  Each
+// thread then takes one of the tile's own agents and walks its stencil out of
+// LDS.  A halo that does not fit the block's LDS budget (very dense cells)
+// falls back to reading the same candidates from global memory, so results
+// never depend on the budget.
+//
+// Per-row narrowing: for each row of cells of the stencil the thread computes,
+// in cell units and conservatively (a 1e-4-cell margin on float arithmetic),
+// the x-interval that can hold agents within the radius, and visits only those
+// cells.  With cells of granularity / 2 this halves the candidates of a 3x3
+// stencil of granularity-sized cells.  The exact near test is applied to
+// every candidate, so narrowing never changes which agents are accepted.
+#pragma once
+
+#include "abl_device.hpp"
+
+namespace abl {
+
+template <int DIM, int TX, int TY, int TZ, int RM>
+struct TileTables {
+  static constexpr int HX = TX + 2 * RM;
+  static constexpr int HY = TY + 2 * RM;
+  static constexpr int HZ = DIM == 3 ? TZ + 2 * RM : 1;
+  static constexpr int MAXROWS = HY * HZ;
+  static constexpr int OWNROWS = TY * (DIM == 3 ? TZ : 1);
+  int h0[3], hn[3], t0[3], tn[3];
+  int nrows, total, own_total, overflow;
+  uint32_t hstart[MAXROWS];            // global slot of each halo row's first agent
+  uint32_t lbase[MAXROWS + 1];         // LDS record index of each halo row's first agent
+  uint32_t ownbase[OWNROWS + 1];       // prefix of the tile's own agents per own row
+  uint32_t ownrow[OWNROWS];            // halo row index of each own row
+  uint32_t lcell[MAXROWS * (HX + 1)];  // LDS record index of each halo cell's start
+
+  __device__ __forceinline__ uint32_t cell_lds(int row, int hx) const {
+    return lcell[row * (HX + 1) + hx];
+  }
+  // global slot of LDS record l in halo row r
+  __device__ __forceinline__ uint32_t global_of(int r, uint32_t l) const {
+    return hstart[r] + (l - lbase[r]);
+  }
+
+  // Block-cooperative set-up for tile `tile` (row-major over the tile grid).
+  // `cap` is the LDS record capacity; overflow is set when the halo exceeds it.
+  __device__ void setup(const GridView &g, int reach, unsigned tile, int cap) {
+    if (threadIdx.x == 0) {
+      const int ntx = (g.n[0] + TX - 1) / TX;
+      const int nty = (g.n[1] + TY - 1) / TY;
+      int b[3];
+      b[0] = (int) (tile % (unsigned) ntx);
+      b[1] = (int) ((tile / (unsigned) ntx) % (unsigned) nty);
+      b[2] = DIM == 3 ? (int) (tile / ((unsigned) ntx * (unsigned) nty)) : 0;
+      const int T[3] = {TX, TY, DIM == 3 ? TZ : 1};
+      for (int d = 0; d < 3; d++) {
+        if (d < DIM) {
+          t0[d] = b[d] * T[d];
+          tn[d] = min(T[d], g.n[d] - t0[d]);
+          h0[d] = max(t0[d] - reach, 0);
+          hn[d] = min(t0[d] + tn[d] + reach, g.n[d]) - h0[d];
+        } else {
+          t0[d] = h0[d] = 0;
+          tn[d] = hn[d] = 1;
+        }
+      }
+      nrows = hn[1] * hn[2];
+    }
+    __syncthreads();
+    const int nx = g.n[0], ny = g.n[1];
+    // halo rows: first slot and agent count
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+      const int hy = r % hn[1], hz = r / hn[1];
+      const uint32_t grow = ((uint32_t) (h0[2] + hz) * (uint32_t) ny + (uint32_t) (h0[1] + hy)) *
+                            (uint32_t) nx;
+      const uint32_t a = g.start[grow + h0[0]];
+      hstart[r] = a;
+      lbase[r + 1] = g.start[grow + h0[0] + hn[0]] - a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      lbase[0] = 0;
+      for (int r = 0; r < nrows; r++) lbase[r + 1] += lbase[r];
+      total = (int) lbase[nrows];
+      overflow = total > cap ? 1 : 0;
+    }
+    __syncthreads();
+    const int w = hn[0] + 1;
+    for (int e = threadIdx.x; e < nrows * w; e += blockDim.x) {
+      const int r = e / w, hx = e % w;
+      const int hy = r % hn[1], hz = r / hn[1];
+      const uint32_t grow = ((uint32_t) (h0[2] + hz) * (uint32_t) ny + (uint32_t) (h0[1] + hy)) *
+                            (uint32_t) nx;
+      lcell[r * (HX + 1) + hx] = lbase[r] + (g.start[grow + h0[0] + hx] - hstart[r]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int oy = t0[1] - h0[1], oz = t0[2] - h0[2], ox = t0[0] - h0[0];
+      const int nq = tn[1] * tn[2];
+      ownbase[0] = 0;
+      for (int q = 0; q < nq; q++) {
+        const int r = (oz + q / tn[1]) * hn[1] + (oy + q % tn[1]);
+        ownrow[q] = (uint32_t) r;
+        ownbase[q + 1] = ownbase[q] + (cell_lds(r, ox + tn[0]) - cell_lds(r, ox));
+      }
+      own_total = (int) ownbase[nq];
+    }
+    __syncthreads();
+  }
+
+  // Halo row of LDS record k (binary search over lbase).
+  __device__ __forceinline__ int row_of_record(uint32_t k) const {
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (lbase[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
+
+  // Own agent k of the tile: its halo row and LDS record index.
+  __device__ __forceinline__ void own(int k, int *row, uint32_t *rec) const {
+    const int nq = tn[1] * tn[2];
+    int lo = 0, hi = nq - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ownbase[mid] <= (uint32_t) k) lo = mid; else hi = mid - 1;
+    }
+    const int r = (int) ownrow[lo];
+    *row = r;
+    *rec = cell_lds(r, t0[0] - h0[0]) + ((uint32_t) k - ownbase[lo]);
+  }
+};
+
+// Conservative x-interval of cells, relative to the agent's own cell, that
+// can hold agents within `rc` cell units of an agent at fractional in-cell
+// position (ux, uy[, uz]) when looking at the row of cells dy (, dz) away.
+// Returns false when the whole row is out of reach.  Positions outside
+// [0, 1) (agents clamped into edge cells) disable narrowing.
+__device__ __forceinline__ bool row_interval(float ux, float uy, float uz, int dy, int dz,
+                                             float rc, int reach, int *xa, int *xb) {
+  const bool inside = ux >= 0.f && ux < 1.f && uy >= 0.f && uy < 1.f && uz >= 0.f && uz < 1.f;
+  if (!inside) {
+    *xa = -reach;
+    *xb = reach;
+    return true;
+  }
+  const float gy = dy > 0 ? (float) dy - uy : (dy < 0 ? uy + (float) (-dy) - 1.f : 0.f);
+  const float gz = dz > 0 ? (float) dz - uz : (dz < 0 ? uz + (float) (-dz) - 1.f : 0.f);
+  const float gap2 = gy * gy + gz * gz;
+  const float r2 = rc * rc;
+  const float margin = 1e-4f;
+  if (gap2 > r2 * (1.f + 1e-5f) + margin) return false;
+  const float w = __builtin_sqrtf(fmaxf(r2 - gap2, 0.f)) + margin;
+  int a = (int) floorf(ux - w);
+  int b = (int) floorf(ux + w);
+  *xa = a < -reach ? -reach : a;
+  *xb = b > reach ? reach : b;
+  return *xa <= *xb;
+}
+
+}  // namespace abl

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "abl_runtime.hpp"
+#include "abl_tile.hpp"
 
 namespace abl {
 namespace {
@@ -33,67 +34,62 @@ struct BoidArgs {
   GridView g;
   float near_t;   // dist <= interaction_radius  <=>  f <= near_t
   float sep_u;    // dist <  separation_radius   <=>  f <  sep_u
+  float rc;       // radius in cell units (row narrowing)
+  int cap;        // LDS records per tile
   int n;
   F time_scale, global_scale, steer, collision, match, max_speed, min_pos, max_pos;
 };
 
-template <typename F, bool BY_ID>
-__global__ __launch_bounds__(1024) void boids_step(BoidArgs<F> a) {
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int i = (int) (blk * blockDim.x + threadIdx.x);
-  if (i >= a.n) return;
-  const F x = a.px[i], y = a.py[i];
-  const F vx = a.vx[i], vy = a.vy[i];
-  const int3 c = cell_of<F>(a.g, x, y, (F) 0);
-
+// Accumulators of the near loop.
+template <typename F>
+struct BoidAcc {
   F gvx = 0, gvy = 0, gcx = 0, gcy = 0, ccx = 0, ccy = 0;
   int icount = 0, ccount = 0;
-  auto body = [&](uint32_t j) {
-    const F qx = a.px[j], qy = a.py[j];
+  __device__ __forceinline__ void visit(const BoidArgs<F> &a, F x, F y, F qx, F qy, F qvx, F qvy) {
     const float f = dist_sq_f32<F>(qx, qy, x, y);
     if (f > a.near_t) return;
     gcx = gcx + qx;
     gcy = gcy + qy;
-    gvx = gvx + a.vx[j];
-    gvy = gvy + a.vy[j];
+    gvx = gvx + qvx;
+    gvy = gvy + qvy;
     icount += 1;
     if (f < a.sep_u) {
       ccx = ccx + qx;
       ccy = ccy + qy;
       ccount += 1;
     }
-  };
-  if (BY_ID) {
-    for_each_candidate_by_id(a.g, c.x, c.y, 0, a.g.reach, a.id, body);
-  } else {
-    for_each_candidate(a.g, c.x, c.y, 0, a.g.reach, body);
   }
+};
 
+// Everything after the near loop, written to slot i.
+template <typename F>
+__device__ __forceinline__ void boid_finish(const BoidArgs<F> &a, BoidAcc<F> s, F x, F y, F vx,
+                                            F vy, uint32_t i) {
   F chx = 0, chy = 0;  // velocity_change
   F sx = 0, sy = 0;    // steer_velocity
-  if (icount > 0) {
-    gcx = gcx / (F) icount;
-    gcy = gcy / (F) icount;
-    sx = (gcx - x) * a.steer;
-    sy = (gcy - y) * a.steer;
+  if (s.icount > 0) {
+    s.gcx = s.gcx / (F) s.icount;
+    s.gcy = s.gcy / (F) s.icount;
+    sx = (s.gcx - x) * a.steer;
+    sy = (s.gcy - y) * a.steer;
   }
   chx = chx + sx;
   chy = chy + sy;
   F mx = 0, my = 0;    // match_velocity
-  if (icount > 0) {
-    gvx = gvx / (F) icount;
-    gvy = gvy / (F) icount;
-    mx = gvx * a.match;
-    my = gvy * a.match;
+  if (s.icount > 0) {
+    s.gvx = s.gvx / (F) s.icount;
+    s.gvy = s.gvy / (F) s.icount;
+    mx = s.gvx * a.match;
+    my = s.gvy * a.match;
   }
   chx = chx + mx;
   chy = chy + my;
   F ax = 0, ay = 0;    // avoid_velocity
-  if (ccount > 0) {
-    ccx = ccx / (F) ccount;
-    ccy = ccy / (F) ccount;
-    ax = (x - ccx) * a.collision;
-    ay = (y - ccy) * a.collision;
+  if (s.ccount > 0) {
+    s.ccx = s.ccx / (F) s.ccount;
+    s.ccy = s.ccy / (F) s.ccount;
+    ax = (x - s.ccx) * a.collision;
+    ay = (y - s.ccy) * a.collision;
   }
   chx = chx + ax;
   chy = chy + ay;
@@ -114,6 +110,118 @@ __global__ __launch_bounds__(1024) void boids_step(BoidArgs<F> a) {
   a.ovy[i] = nvy;
 }
 
+// Untiled: one thread per agent, candidates read from global memory (cell
+// order), or merged into the reference's id order (BY_ID).
+template <typename F, bool BY_ID>
+__global__ __launch_bounds__(1024) void boids_step(BoidArgs<F> a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  const F x = a.px[i], y = a.py[i];
+  const int3 c = cell_of<F>(a.g, x, y, (F) 0);
+  BoidAcc<F> s;
+  auto body = [&](uint32_t j) { s.visit(a, x, y, a.px[j], a.py[j], a.vx[j], a.vy[j]); };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, 0, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, 0, a.g.reach, body);
+  }
+  boid_finish<F>(a, s, x, y, a.vx[i], a.vy[i], (uint32_t) i);
+}
+
+// LDS-tiled: one block per TX x TY tile of cells, halo records
+// {x, y, vx, vy} staged in LDS, per-row narrowed stencils.  Visits the same
+// accepted neighbours in the same order as boids_step<F, false>.
+template <typename F, int TX, int TY, int RM>
+__global__ __launch_bounds__(256) void boids_tile(BoidArgs<F> a) {
+  using TT = TileTables<2, TX, TY, 1, RM>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  TT &T = *reinterpret_cast<TT *>(smem);
+  F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
+  const int R = a.g.reach;
+  T.setup(a.g, R, xcd_block(blockIdx.x, gridDim.x), a.cap);
+  const bool ovf = T.overflow != 0;
+  if (!ovf) {
+    for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
+      const uint32_t gi = T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k);
+      rec[4 * k + 0] = a.px[gi];
+      rec[4 * k + 1] = a.py[gi];
+      rec[4 * k + 2] = a.vx[gi];
+      rec[4 * k + 3] = a.vy[gi];
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < T.own_total; k += blockDim.x) {
+    int orow;
+    uint32_t ol;
+    T.own(k, &orow, &ol);
+    const uint32_t gi = T.global_of(orow, ol);
+    F x, y, vx, vy;
+    if (!ovf) {
+      x = rec[4 * ol];
+      y = rec[4 * ol + 1];
+      vx = rec[4 * ol + 2];
+      vy = rec[4 * ol + 3];
+    } else {
+      x = a.px[gi];
+      y = a.py[gi];
+      vx = a.vx[gi];
+      vy = a.vy[gi];
+    }
+    const double qx = ((double) x - a.g.lo[0]) * a.g.inv_cell;
+    const double qy = ((double) y - a.g.lo[1]) * a.g.inv_cell;
+    const int3 c = cell_of<F>(a.g, x, y, (F) 0);
+    const float ux = (float) (qx - (double) c.x), uy = (float) (qy - (double) c.y);
+    BoidAcc<F> s;
+    for (int dy = -R; dy <= R; dy++) {
+      const int yy = c.y + dy;
+      if (yy < T.h0[1] || yy >= T.h0[1] + T.hn[1]) continue;
+      int xa, xb;
+      if (!row_interval(ux, uy, 0.f, dy, 0, a.rc, R, &xa, &xb)) continue;
+      const int xlo = max(c.x + xa, T.h0[0]);
+      const int xhi = min(c.x + xb, T.h0[0] + T.hn[0] - 1);
+      if (xlo > xhi) continue;
+      const int r = yy - T.h0[1];
+      const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
+      const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
+      if (!ovf) {
+        for (uint32_t l = l0; l < l1; l++) {
+          s.visit(a, x, y, rec[4 * l], rec[4 * l + 1], rec[4 * l + 2], rec[4 * l + 3]);
+        }
+      } else {
+        const uint32_t g0 = T.global_of(r, l0);
+        for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
+          s.visit(a, x, y, a.px[j], a.py[j], a.vx[j], a.vy[j]);
+        }
+      }
+    }
+    boid_finish<F>(a, s, x, y, vx, vy, gi);
+  }
+}
+
+template <typename F, int TX, int TY, int RM>
+void launch_tile(const StepLaunch &L, const BoidArgs<F> &a0) {
+  using TT = TileTables<2, TX, TY, 1, RM>;
+  BoidArgs<F> a = a0;
+  const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
+  const double halo = (double) (TX + 2 * RM) * (TY + 2 * RM) * (double) L.occupancy;
+  const size_t budget = 64 * 1024 - head;
+  int cap = (int) (2.0 * halo) + 128;
+  const int max_cap = (int) (budget / (4 * sizeof(F)));
+  a.cap = cap < max_cap ? cap : max_cap;
+  const size_t smem = head + (size_t) a.cap * 4 * sizeof(F);
+  const unsigned ntiles = (unsigned) (((L.grid.n[0] + TX - 1) / TX) * ((L.grid.n[1] + TY - 1) / TY));
+  const int threads = L.block > 256 ? 256 : L.block;
+  hipLaunchKernelGGL((boids_tile<F, TX, TY, RM>), ntiles, threads, smem, L.stream, a);
+}
+
+template <typename F, int TX, int TY>
+void launch_tile_r(const StepLaunch &L, const BoidArgs<F> &a) {
+  if (L.grid.reach <= 1) launch_tile<F, TX, TY, 1>(L, a);
+  else if (L.grid.reach == 2) launch_tile<F, TX, TY, 2>(L, a);
+  else launch_tile<F, TX, TY, 4>(L, a);
+}
+
 template <typename F>
 void launch_typed(const StepLaunch &L) {
   BoidArgs<F> a{};
@@ -129,6 +237,7 @@ void launch_typed(const StepLaunch &L) {
   a.g = L.grid;
   a.near_t = L.near_t;
   a.sep_u = sqrtf_lt_threshold((double) (F) L.params[P_SEPARATION_RADIUS]);
+  a.rc = L.rc;
   a.n = L.n;
   a.time_scale = (F) L.params[P_TIME_SCALE];
   a.global_scale = (F) L.params[P_GLOBAL_SCALE];
@@ -138,11 +247,16 @@ void launch_typed(const StepLaunch &L) {
   a.max_speed = (F) L.params[P_MAX_SPEED];
   a.min_pos = (F) L.params[P_MIN_POS];
   a.max_pos = (F) L.params[P_MAX_POS];
-  const unsigned nb = grid_blocks(L.n, L.block);
   if (L.order == 1) {
-    hipLaunchKernelGGL((boids_step<F, true>), nb, L.block, 0, L.stream, a);
+    hipLaunchKernelGGL((boids_step<F, true>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
+  } else if (L.tile == 1 && L.grid.reach <= 4) {
+    launch_tile_r<F, 16, 8>(L, a);
+  } else if (L.tile == 2 && L.grid.reach <= 4) {
+    launch_tile_r<F, 8, 8>(L, a);
+  } else if (L.tile == 3 && L.grid.reach <= 4) {
+    launch_tile_r<F, 32, 8>(L, a);
   } else {
-    hipLaunchKernelGGL((boids_step<F, false>), nb, L.block, 0, L.stream, a);
+    hipLaunchKernelGGL((boids_step<F, false>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   }
 }
 
