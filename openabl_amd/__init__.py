@@ -7,8 +7,8 @@ as cell-sorted struct-of-arrays in HBM and runs the per-timestep update of an
 kernels behind the C ABI of ``include/abl_hip.h``.  :mod:`openabl_amd.runtime`
 binds that ABI for Python callers.
 """
-from .runtime import (AGENT_NAMES, MEMBERS, AblError, Simulation, derived_params, fold6,
+from .runtime import (AGENT_NAMES, AGENTS, MEMBERS, AblError, Simulation, derived_params, fold6,
                       load_library, record_dtype)
 
-__all__ = ["AGENT_NAMES", "MEMBERS", "AblError", "Simulation", "derived_params", "fold6",
-           "load_library", "record_dtype"]
+__all__ = ["AGENT_NAMES", "AGENTS", "MEMBERS", "AblError", "Simulation", "derived_params",
+           "fold6", "load_library", "record_dtype"]

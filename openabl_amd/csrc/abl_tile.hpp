@@ -151,15 +151,120 @@ __device__ __forceinline__ bool row_interval(float ux, float uy, float uz, int d
   const float gy = dy > 0 ? (float) dy - uy : (dy < 0 ? uy + (float) (-dy) - 1.f : 0.f);
   const float gz = dz > 0 ? (float) dz - uz : (dz < 0 ? uz + (float) (-dz) - 1.f : 0.f);
   const float gap2 = gy * gy + gz * gz;
-  const float r2 = rc * rc;
+  // radius squared, enlarged well past the float error of gap2 (values <= ~16,
+  // error ~1e-6) so every agent the exact test can accept stays inside
   const float margin = 1e-4f;
-  if (gap2 > r2 * (1.f + 1e-5f) + margin) return false;
-  const float w = __builtin_sqrtf(fmaxf(r2 - gap2, 0.f)) + margin;
+  const float rr2 = rc * rc * (1.f + 1e-5f) + margin;
+  if (gap2 > rr2) return false;
+  const float w = __builtin_sqrtf(rr2 - gap2) + margin;
   int a = (int) floorf(ux - w);
   int b = (int) floorf(ux + w);
   *xa = a < -reach ? -reach : a;
   *xb = b > reach ? reach : b;
   return *xa <= *xb;
+}
+
+}  // namespace abl
+
+namespace abl {
+
+// Generic LDS-tiled step kernel for a step whose near loop runs over its own
+// agent type.  The policy M supplies:
+//   using F; static constexpr int DIM, NS;   NS = abl_float values per record
+//   struct Args  (with GridView g; float rc; int cap;)
+//   static void load(const Args &, uint32_t slot, F *rec)     -- NS values
+//   struct Acc   { void visit(const Args &, const F *own, const F *nb); }
+//   static void begin(const Args &, Acc &, const F *own)      -- before the loop
+//   static void finish(const Args &, Acc &, const F *own, uint32_t slot)
+// Records are the first DIM values = position, then the members the near
+// loop reads.  Candidates are visited row by row of the (narrowed) stencil,
+// each row in cell-sorted order: the same order as for_each_candidate.
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_step(typename M::Args a) {
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  using TT = TileTables<DIM, TX, TY, TZ, RM>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  TT &T = *reinterpret_cast<TT *>(smem);
+  F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
+  const GridView &g = a.g;
+  const int R = g.reach;
+  T.setup(g, R, xcd_block(blockIdx.x, gridDim.x), a.cap);
+  const bool ovf = T.overflow != 0;
+  if (!ovf) {
+    for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
+      M::load(a, T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k), rec + (size_t) NS * k);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < T.own_total; k += blockDim.x) {
+    int orow;
+    uint32_t ol;
+    T.own(k, &orow, &ol);
+    const uint32_t gi = T.global_of(orow, ol);
+    F own[NS];
+    if (!ovf) {
+#pragma unroll
+      for (int v = 0; v < NS; v++) own[v] = rec[(size_t) NS * ol + v];
+    } else {
+      M::load(a, gi, own);
+    }
+    const int3 c = cell_of<F>(g, own[0], own[1], DIM == 3 ? own[2] : (F) 0);
+    const float ux = (float) (((double) own[0] - g.lo[0]) * g.inv_cell - (double) c.x);
+    const float uy = (float) (((double) own[1] - g.lo[1]) * g.inv_cell - (double) c.y);
+    const float uz = DIM == 3 ? (float) (((double) own[2] - g.lo[2]) * g.inv_cell - (double) c.z) : 0.f;
+    typename M::Acc acc;
+    M::begin(a, acc, own);
+    const int zr = DIM == 3 ? R : 0;
+    for (int dz = -zr; dz <= zr; dz++) {
+      const int zz = c.z + dz;
+      if (zz < T.h0[2] || zz >= T.h0[2] + T.hn[2]) continue;
+      for (int dy = -R; dy <= R; dy++) {
+        const int yy = c.y + dy;
+        if (yy < T.h0[1] || yy >= T.h0[1] + T.hn[1]) continue;
+        int xa, xb;
+        if (!row_interval(ux, uy, uz, dy, dz, a.rc, R, &xa, &xb)) continue;
+        const int xlo = max(c.x + xa, T.h0[0]);
+        const int xhi = min(c.x + xb, T.h0[0] + T.hn[0] - 1);
+        if (xlo > xhi) continue;
+        const int r = (zz - T.h0[2]) * T.hn[1] + (yy - T.h0[1]);
+        const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
+        const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
+        if (!ovf) {
+          for (uint32_t l = l0; l < l1; l++) acc.visit(a, own, rec + (size_t) NS * l);
+        } else {
+          const uint32_t g0 = T.global_of(r, l0);
+          for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
+            F nb[NS];
+            M::load(a, j, nb);
+            acc.visit(a, own, nb);
+          }
+        }
+      }
+    }
+    M::finish(a, acc, own, gi);
+  }
+}
+
+// Host side: LDS budget and launch of tiled_step for a tile shape.
+template <class M, int TX, int TY, int TZ, int RM>
+inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy, int block,
+                         hipStream_t stream) {
+  using F = typename M::F;
+  using TT = TileTables<M::DIM, TX, TY, TZ, RM>;
+  const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
+  const double halo_cells = (double) (TX + 2 * RM) * (TY + 2 * RM) *
+                            (M::DIM == 3 ? (double) (TZ + 2 * RM) : 1.0);
+  const size_t budget = 64 * 1024 - head;
+  const int max_cap = (int) (budget / (M::NS * sizeof(F)));
+  const int want = (int) (2.0 * halo_cells * (double) occupancy) + 128;
+  a.cap = want < max_cap ? want : max_cap;
+  const size_t smem = head + (size_t) a.cap * M::NS * sizeof(F);
+  unsigned ntiles = (unsigned) (((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY));
+  if (M::DIM == 3) ntiles *= (unsigned) ((g.n[2] + TZ - 1) / TZ);
+  const int threads = block > 256 ? 256 : block;
+  hipLaunchKernelGGL((tiled_step<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a);
 }
 
 }  // namespace abl

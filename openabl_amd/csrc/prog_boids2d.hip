@@ -129,97 +129,36 @@ __global__ __launch_bounds__(1024) void boids_step(BoidArgs<F> a) {
   boid_finish<F>(a, s, x, y, a.vx[i], a.vy[i], (uint32_t) i);
 }
 
-// LDS-tiled: one block per TX x TY tile of cells, halo records
-// {x, y, vx, vy} staged in LDS, per-row narrowed stencils.  Visits the same
+// LDS-tiled policy (abl_tile.hpp): records {x, y, vx, vy}; visits the same
 // accepted neighbours in the same order as boids_step<F, false>.
-template <typename F, int TX, int TY, int RM>
-__global__ __launch_bounds__(256) void boids_tile(BoidArgs<F> a) {
-  using TT = TileTables<2, TX, TY, 1, RM>;
-  extern __shared__ __align__(16) unsigned char smem[];
-  TT &T = *reinterpret_cast<TT *>(smem);
-  F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
-  const int R = a.g.reach;
-  T.setup(a.g, R, xcd_block(blockIdx.x, gridDim.x), a.cap);
-  const bool ovf = T.overflow != 0;
-  if (!ovf) {
-    for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
-      const uint32_t gi = T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k);
-      rec[4 * k + 0] = a.px[gi];
-      rec[4 * k + 1] = a.py[gi];
-      rec[4 * k + 2] = a.vx[gi];
-      rec[4 * k + 3] = a.vy[gi];
-    }
+template <typename F_>
+struct BoidPolicy {
+  using F = F_;
+  static constexpr int DIM = 2, NS = 4;
+  using Args = BoidArgs<F>;
+  __device__ static void load(const Args &a, uint32_t j, F *r) {
+    r[0] = a.px[j];
+    r[1] = a.py[j];
+    r[2] = a.vx[j];
+    r[3] = a.vy[j];
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < T.own_total; k += blockDim.x) {
-    int orow;
-    uint32_t ol;
-    T.own(k, &orow, &ol);
-    const uint32_t gi = T.global_of(orow, ol);
-    F x, y, vx, vy;
-    if (!ovf) {
-      x = rec[4 * ol];
-      y = rec[4 * ol + 1];
-      vx = rec[4 * ol + 2];
-      vy = rec[4 * ol + 3];
-    } else {
-      x = a.px[gi];
-      y = a.py[gi];
-      vx = a.vx[gi];
-      vy = a.vy[gi];
+  struct Acc : BoidAcc<F> {
+    __device__ void visit(const Args &a, const F *own, const F *nb) {
+      BoidAcc<F>::visit(a, own[0], own[1], nb[0], nb[1], nb[2], nb[3]);
     }
-    const double qx = ((double) x - a.g.lo[0]) * a.g.inv_cell;
-    const double qy = ((double) y - a.g.lo[1]) * a.g.inv_cell;
-    const int3 c = cell_of<F>(a.g, x, y, (F) 0);
-    const float ux = (float) (qx - (double) c.x), uy = (float) (qy - (double) c.y);
-    BoidAcc<F> s;
-    for (int dy = -R; dy <= R; dy++) {
-      const int yy = c.y + dy;
-      if (yy < T.h0[1] || yy >= T.h0[1] + T.hn[1]) continue;
-      int xa, xb;
-      if (!row_interval(ux, uy, 0.f, dy, 0, a.rc, R, &xa, &xb)) continue;
-      const int xlo = max(c.x + xa, T.h0[0]);
-      const int xhi = min(c.x + xb, T.h0[0] + T.hn[0] - 1);
-      if (xlo > xhi) continue;
-      const int r = yy - T.h0[1];
-      const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
-      const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
-      if (!ovf) {
-        for (uint32_t l = l0; l < l1; l++) {
-          s.visit(a, x, y, rec[4 * l], rec[4 * l + 1], rec[4 * l + 2], rec[4 * l + 3]);
-        }
-      } else {
-        const uint32_t g0 = T.global_of(r, l0);
-        for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
-          s.visit(a, x, y, a.px[j], a.py[j], a.vx[j], a.vy[j]);
-        }
-      }
-    }
-    boid_finish<F>(a, s, x, y, vx, vy, gi);
+  };
+  __device__ static void begin(const Args &, Acc &, const F *) {}
+  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t gi) {
+    boid_finish<F>(a, s, own[0], own[1], own[2], own[3], gi);
   }
-}
-
-template <typename F, int TX, int TY, int RM>
-void launch_tile(const StepLaunch &L, const BoidArgs<F> &a0) {
-  using TT = TileTables<2, TX, TY, 1, RM>;
-  BoidArgs<F> a = a0;
-  const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
-  const double halo = (double) (TX + 2 * RM) * (TY + 2 * RM) * (double) L.occupancy;
-  const size_t budget = 64 * 1024 - head;
-  int cap = (int) (2.0 * halo) + 128;
-  const int max_cap = (int) (budget / (4 * sizeof(F)));
-  a.cap = cap < max_cap ? cap : max_cap;
-  const size_t smem = head + (size_t) a.cap * 4 * sizeof(F);
-  const unsigned ntiles = (unsigned) (((L.grid.n[0] + TX - 1) / TX) * ((L.grid.n[1] + TY - 1) / TY));
-  const int threads = L.block > 256 ? 256 : L.block;
-  hipLaunchKernelGGL((boids_tile<F, TX, TY, RM>), ntiles, threads, smem, L.stream, a);
-}
+};
 
 template <typename F, int TX, int TY>
 void launch_tile_r(const StepLaunch &L, const BoidArgs<F> &a) {
-  if (L.grid.reach <= 1) launch_tile<F, TX, TY, 1>(L, a);
-  else if (L.grid.reach == 2) launch_tile<F, TX, TY, 2>(L, a);
-  else launch_tile<F, TX, TY, 4>(L, a);
+  using M = BoidPolicy<F>;
+  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream);
+  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream);
+  else launch_tiled<M, TX, TY, 1, 4>(a, L.grid, L.occupancy, L.block, L.stream);
 }
 
 template <typename F>

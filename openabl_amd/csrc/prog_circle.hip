@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "abl_runtime.hpp"
+#include "abl_tile.hpp"
 
 namespace abl {
 namespace {
@@ -39,53 +40,40 @@ struct CircleArgs {
   const int32_t *id;
   GridView g;
   float near_t;
+  float rc;        // radius in cell units (row narrowing)
+  int cap;         // LDS records per tile
   int n;
   F r, k_rep, k_att, W;
 };
 
-template <typename F, int DIM, bool BY_ID>
-__global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int i = (int) (blk * blockDim.x + threadIdx.x);
-  if (i >= a.n) return;
-  F x[3];
-  F np[3];
+// One candidate of the near loop: new_pos += force (or skip).
+template <typename F, int DIM>
+__device__ __forceinline__ void circle_visit(const CircleArgs<F, DIM> &a, const F *x, const F *q,
+                                             F *np) {
+  // near test: dist(nx.pos, in.pos) > 2.0*r -> continue
+  const float f = DIM == 3 ? dist_sq_f32<F>(q[0], q[1], q[2], x[0], x[1], x[2])
+                           : dist_sq_f32<F>(q[0], q[1], x[0], x[1]);
+  if (f > a.near_t) return;
+  // pos_dist = dist(in.pos, nx.pos): same sum of squares (negated terms)
+  const F pos_dist = (F) sqrt_rn_f32(f);
+  if (pos_dist == (F) 0) return;
+  const F sep_dist = pos_dist - a.r;
+  F force[3];
+  if ((double) sep_dist > 0.0) {
+    const F s = (F) ((double) a.k_att * ((2.0 * (double) a.r) - (double) pos_dist));
 #pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    x[d] = a.p[d][i];
-    np[d] = x[d];
-  }
-  const int3 c = cell_of<F>(a.g, x[0], x[1], DIM == 3 ? x[2] : (F) 0);
-  const F r = a.r;
-  auto body = [&](uint32_t j) {
-    F q[3];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) q[d] = a.p[d][j];
-    // near test: dist(nx.pos, in.pos) > 2.0*r -> continue
-    const float f = DIM == 3 ? dist_sq_f32<F>(q[0], q[1], q[2], x[0], x[1], x[2])
-                             : dist_sq_f32<F>(q[0], q[1], x[0], x[1]);
-    if (f > a.near_t) return;
-    // pos_dist = dist(in.pos, nx.pos): same sum of squares (negated terms)
-    const F pos_dist = (F) sqrt_rn_f32(f);
-    if (pos_dist == (F) 0) return;
-    const F sep_dist = pos_dist - r;
-    F force[3];
-    if ((double) sep_dist > 0.0) {
-      const F s = (F) ((double) a.k_att * ((2.0 * (double) r) - (double) pos_dist));
-#pragma unroll
-      for (int d = 0; d < DIM; d++) force[d] = ((q[d] - x[d]) * s) / pos_dist;
-    } else {
-#pragma unroll
-      for (int d = 0; d < DIM; d++) force[d] = (x[d] - q[d]) * a.k_rep;
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; d++) np[d] = np[d] + force[d];
-  };
-  if (BY_ID) {
-    for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
+    for (int d = 0; d < DIM; d++) force[d] = ((q[d] - x[d]) * s) / pos_dist;
   } else {
-    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) force[d] = (x[d] - q[d]) * a.k_rep;
   }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) np[d] = np[d] + force[d];
+}
+
+// out.pos = clamp(new_pos, float(1), float(W - 1.0))
+template <typename F, int DIM>
+__device__ __forceinline__ void circle_finish(const CircleArgs<F, DIM> &a, const F *np, uint32_t i) {
   const F lo = (F) 1;
   const F hi = (F) ((double) a.W - 1.0);
 #pragma unroll
@@ -93,6 +81,66 @@ __global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
     const F v = np[d];
     a.o[d][i] = v < lo ? lo : (v > hi ? hi : v);
   }
+}
+
+template <typename F, int DIM, bool BY_ID>
+__global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  F x[3] = {0, 0, 0};
+  F np[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    x[d] = a.p[d][i];
+    np[d] = x[d];
+  }
+  const int3 c = cell_of<F>(a.g, x[0], x[1], DIM == 3 ? x[2] : (F) 0);
+  auto body = [&](uint32_t j) {
+    F q[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) q[d] = a.p[d][j];
+    circle_visit<F, DIM>(a, x, q, np);
+  };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+  }
+  circle_finish<F, DIM>(a, np, (uint32_t) i);
+}
+
+// LDS-tiled policy (abl_tile.hpp): records are the positions.
+template <typename F_, int DIM_>
+struct CirclePolicy {
+  using F = F_;
+  static constexpr int DIM = DIM_, NS = DIM_;
+  using Args = CircleArgs<F, DIM>;
+  __device__ static void load(const Args &a, uint32_t j, F *r) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) r[d] = a.p[d][j];
+  }
+  struct Acc {
+    F np[DIM];
+    __device__ void visit(const Args &a, const F *own, const F *nb) {
+      circle_visit<F, DIM>(a, own, nb, np);
+    }
+  };
+  __device__ static void begin(const Args &, Acc &s, const F *own) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.np[d] = own[d];
+  }
+  __device__ static void finish(const Args &a, Acc &s, const F *, uint32_t i) {
+    circle_finish<F, DIM>(a, s.np, i);
+  }
+};
+
+template <typename F, int DIM, int TX, int TY, int TZ>
+void launch_tile_r(const StepLaunch &L, const CircleArgs<F, DIM> &a) {
+  using M = CirclePolicy<F, DIM>;
+  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream);
+  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream);
+  else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream);
 }
 
 template <typename F, int DIM>
@@ -105,6 +153,7 @@ void launch_typed(const StepLaunch &L) {
   a.id = L.nb_id;
   a.g = L.grid;
   a.near_t = L.near_t;
+  a.rc = L.rc;
   a.n = L.n;
   a.r = (F) L.params[P_R];
   a.k_rep = (F) L.params[P_K_REP];
@@ -113,6 +162,11 @@ void launch_typed(const StepLaunch &L) {
   const unsigned nb = grid_blocks(L.n, L.block);
   if (L.order == 1) {
     hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
+  } else if (L.tile != 0 && L.grid.reach <= 4) {
+    if (DIM == 3) launch_tile_r<F, DIM, 8, 4, 4>(L, a);
+    else if (L.tile == 2) launch_tile_r<F, DIM, 8, 8, 1>(L, a);
+    else if (L.tile == 3) launch_tile_r<F, DIM, 32, 8, 1>(L, a);
+    else launch_tile_r<F, DIM, 16, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((circle_step<F, DIM, false>), nb, L.block, 0, L.stream, a);
   }
