@@ -161,6 +161,12 @@ int abl_hip_download_all(abl_sim *sim, void *agents, const abl_agent_info *info)
 int abl_hip_simulate(abl_sim *sim, int timesteps);
 /* Wait for the sim's stream. */
 int abl_hip_synchronize(abl_sim *sim);
+/* Non-blocking: *done = 1 when all work queued on the sim has finished. */
+int abl_hip_query(abl_sim *sim, int *done);
+/* Device bytes a simulation of `program` holding `agents` agents over `cells`
+ * grid cells would need (admission control for serving). */
+int abl_hip_estimate_footprint(const char *program, int precision, size_t agents, size_t cells,
+                               size_t *bytes);
 
 /* Parity probe: the (centre id, neighbour id) pairs that step function
  * `step`'s near loop accepts on the current state, sorted by centre then
@@ -190,6 +196,47 @@ int abl_hip_reduce_count_eq(abl_sim *sim, int agent_type, const char *member, do
 int abl_hip_last_exec_time(abl_sim *sim, double *seconds);
 int abl_hip_log_csv(abl_sim *sim, const char *row);
 int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len);
+
+/* ---- serving: continuous batching of runs (no counterpart in the
+ * reference, which runs one simulation per process, src/main.cpp:280-311) --
+ * Requests arrive at any time; a worker thread admits them in arrival order
+ * while the estimated device footprint of the active runs fits `hbm_budget`
+ * and at most `max_active` run at once, initialises each (serialised: the
+ * generator is process-wide) from the request's generator state, and advances
+ * every active run `chunk_steps` timesteps at a time on its own stream, so
+ * runs overlap on the device.  Finished runs are downloaded (reference array
+ * order) and their memory released.  Results do not depend on the mix. */
+typedef struct abl_server abl_server;
+
+typedef struct {
+  const char *program;
+  int precision;
+  int num_params;
+  const char *const *param_names;  /* -P overrides, derived constants folded */
+  const double *param_values;
+  int timesteps;
+  uint64_t rng_state[2];           /* generator state the run's init starts from */
+  size_t agents_hint;              /* most agents the run holds (admission) */
+  size_t cells_hint;               /* grid cells of its environment (admission) */
+} abl_run_request;
+
+enum { ABL_RUN_QUEUED = 0, ABL_RUN_ACTIVE = 1, ABL_RUN_DONE = 2, ABL_RUN_FAILED = 3 };
+
+const char *abl_server_last_error(void);
+int abl_server_create(size_t hbm_budget, int max_active, int chunk_steps, abl_server **out);
+int abl_server_submit(abl_server *srv, const abl_run_request *req, uint64_t *run_id);
+int abl_server_status(abl_server *srv, uint64_t run_id, int *state);
+int abl_server_wait(abl_server *srv, uint64_t run_id);
+int abl_server_result(abl_server *srv, uint64_t run_id, int agent_type, void *aos,
+                      size_t capacity, size_t *count);
+int abl_server_run_log(abl_server *srv, uint64_t run_id, char *buf, size_t capacity, size_t *len);
+/* seconds: [0] queued (submit -> admit), [1] on device (admit -> done), [2] total */
+int abl_server_run_times(abl_server *srv, uint64_t run_id, double out[3]);
+/* [0] completed runs, [1] p50 and [2] p99 latency (submit -> result, s),
+ * [3] agent-steps completed, [4] seconds from first admission to last
+ * completion, [5] peak admitted footprint (bytes) */
+int abl_server_stats(abl_server *srv, double out[6]);
+int abl_server_destroy(abl_server *srv);
 
 /* Device memory the sim holds, in bytes. */
 int abl_hip_footprint(abl_sim *sim, size_t *bytes);

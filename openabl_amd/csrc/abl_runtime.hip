@@ -1189,6 +1189,43 @@ extern "C" int abl_hip_synchronize(abl_sim *sim) {
   return ABL_OK;
 }
 
+extern "C" int abl_hip_query(abl_sim *sim, int *done) {
+  CHECK_SIM(sim);
+  if (!done) return fail(ABL_ERR_ARG, "null argument");
+  hipError_t e = hipStreamQuery(sim->stream);
+  if (e == hipSuccess) {
+    *done = 1;
+    return ABL_OK;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return ABL_OK;
+  }
+  return fail(ABL_ERR_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+// Device bytes a simulation of `program` with `agents` agents in total would
+// hold (columns, ids, grid scratch, staging), for admission control.
+extern "C" int abl_hip_estimate_footprint(const char *program, int precision, size_t agents,
+                                          size_t cells, size_t *bytes) {
+  if (!program || !bytes) return fail(ABL_ERR_ARG, "null argument");
+  const ProgramDesc *prog = nullptr;
+  for (int i = 0; programs()[i]; i++) {
+    if (std::strcmp(programs()[i]->name, program) == 0) prog = programs()[i];
+  }
+  if (!prog) return fail(ABL_ERR_ARG, std::string("unknown program ") + program);
+  size_t per_agent = 0;
+  for (const AgentDesc &d : prog->agents) {
+    AgentStore s;
+    layout_agent(s, d, precision);
+    size_t cols = 0;
+    for (const Column &c : s.cols) cols += (size_t) c.elem;
+    per_agent = std::max(per_agent, 2 * cols + 8 + 12 + s.aos_size + 2 * cols);
+  }
+  *bytes = agents * per_agent + cells * 8 * prog->agents.size() + (1u << 20);
+  return ABL_OK;
+}
+
 extern "C" int abl_hip_footprint(abl_sim *sim, size_t *bytes) {
   CHECK_SIM(sim);
   if (!bytes) return fail(ABL_ERR_ARG, "null argument");

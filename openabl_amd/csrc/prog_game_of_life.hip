@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "abl_runtime.hpp"
+#include "abl_tile.hpp"
 
 namespace abl {
 namespace {
@@ -27,8 +28,21 @@ struct LifeArgs {
   uint8_t *out_alive;
   GridView g;
   float near_t;
+  float rc;
+  int cap;
   int n;
 };
+
+__device__ __forceinline__ uint8_t life_rule(bool alive, int living) {
+  bool out;
+  if (alive) {
+    living -= 1;
+    out = living >= 2 && living <= 3;
+  } else {
+    out = living == 3;
+  }
+  return out ? 1 : 0;
+}
 
 template <typename F>
 __global__ __launch_bounds__(1024) void life_step(LifeArgs<F> a) {
@@ -43,15 +57,32 @@ __global__ __launch_bounds__(1024) void life_step(LifeArgs<F> a) {
     if (f > a.near_t) return;
     living += a.alive[j] ? 1 : 0;
   });
-  bool out;
-  if (a.alive[i]) {
-    living -= 1;
-    out = living >= 2 && living <= 3;
-  } else {
-    out = living == 3;
-  }
-  a.out_alive[i] = out ? 1 : 0;
+  a.out_alive[i] = life_rule(a.alive[i] != 0, living);
 }
+
+// LDS-tiled policy (abl_tile.hpp): records {x, y, alive as 0/1}.
+template <typename F_>
+struct LifePolicy {
+  using F = F_;
+  static constexpr int DIM = 2, NS = 3;
+  using Args = LifeArgs<F>;
+  __device__ static void load(const Args &a, uint32_t j, F *r) {
+    r[0] = a.px[j];
+    r[1] = a.py[j];
+    r[2] = a.alive[j] ? (F) 1 : (F) 0;
+  }
+  struct Acc {
+    int living;
+    __device__ void visit(const Args &a, const F *own, const F *nb) {
+      if (dist_sq_f32<F>(nb[0], nb[1], own[0], own[1]) > a.near_t) return;
+      living += nb[2] != (F) 0 ? 1 : 0;
+    }
+  };
+  __device__ static void begin(const Args &, Acc &s, const F *) { s.living = 0; }
+  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t i) {
+    a.out_alive[i] = life_rule(own[2] != (F) 0, s.living);
+  }
+};
 
 template <typename F>
 void launch_typed(const StepLaunch &L) {
@@ -62,8 +93,16 @@ void launch_typed(const StepLaunch &L) {
   a.out_alive = (uint8_t *) L.out.c[2];
   a.g = L.grid;
   a.near_t = L.near_t;
+  a.rc = L.rc;
   a.n = L.n;
-  hipLaunchKernelGGL(life_step<F>, grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
+  using M = LifePolicy<F>;
+  if (L.tile != 0 && L.grid.reach == 1) {
+    launch_tiled<M, 16, 8, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream);
+  } else if (L.tile != 0 && L.grid.reach == 2) {
+    launch_tiled<M, 16, 8, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream);
+  } else {
+    hipLaunchKernelGGL(life_step<F>, grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
+  }
 }
 
 // main() of game_of_life.abl before simulate:

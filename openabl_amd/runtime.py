@@ -63,6 +63,18 @@ _SIGNATURES = {
     "abl_hip_last_exec_time": ([C.c_void_p, C.c_void_p], C.c_int),
     "abl_hip_log_csv": ([C.c_void_p, C.c_char_p], C.c_int),
     "abl_hip_log_text": ([C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_hip_query": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_hip_estimate_footprint": ([C.c_char_p, C.c_int, C.c_size_t, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_server_last_error": ([], C.c_char_p),
+    "abl_server_create": ([C.c_size_t, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "abl_server_submit": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "abl_server_status": ([C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
+    "abl_server_wait": ([C.c_void_p, C.c_uint64], C.c_int),
+    "abl_server_result": ([C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_server_run_log": ([C.c_void_p, C.c_uint64, C.c_char_p, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_server_run_times": ([C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
+    "abl_server_stats": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_server_destroy": ([C.c_void_p], C.c_int),
 }
 
 
