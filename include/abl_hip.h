@@ -197,6 +197,30 @@ int abl_hip_last_exec_time(abl_sim *sim, double *seconds);
 int abl_hip_log_csv(abl_sim *sim, const char *row);
 int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len);
 
+/* ---- batched ensembles (no counterpart in the reference) -----------------
+ * `members` independent runs of one program (uneven agent counts; parameters
+ * per member, row-major members x num_params) advanced together: one counting
+ * sort and one tiled step launch cover every member each timestep, and the
+ * timestep is replayed as a captured HIP graph ("hip.graph", default on).
+ * Member m's initialisation starts from abl_ens_member_seed(s0, s1, m); each
+ * member's result is bit-identical to a run of it alone from that state with
+ * the same "hip.cell_scale" and "hip.tile".  Programs: boids2d. */
+typedef struct abl_ensemble abl_ensemble;
+const char *abl_ens_last_error(void);
+int abl_ens_member_seed(uint64_t s0, uint64_t s1, int member, uint64_t out[2]);
+int abl_ens_create(const char *program, int precision, int members, const int *counts,
+                   int num_params, const char *const *param_names, const double *param_values,
+                   abl_ensemble **out);
+int abl_ens_set_config(abl_ensemble *ens, const char *key, const char *value);
+int abl_ens_init(abl_ensemble *ens, uint64_t s0, uint64_t s1);
+int abl_ens_simulate(abl_ensemble *ens, int timesteps);
+/* waits; *ms receives the device time of the last simulate call */
+int abl_ens_synchronize(abl_ensemble *ens, double *ms);
+int abl_ens_member_count(abl_ensemble *ens, int member, size_t *count);
+int abl_ens_agents(abl_ensemble *ens, size_t *total);
+int abl_ens_download(abl_ensemble *ens, int member, void *aos, size_t capacity, size_t *count);
+int abl_ens_destroy(abl_ensemble *ens);
+
 /* ---- serving: continuous batching of runs (no counterpart in the
  * reference, which runs one simulation per process, src/main.cpp:280-311) --
  * Requests arrive at any time; a worker thread admits them in arrival order

@@ -32,6 +32,7 @@
 
 #include "../../include/abl_hip.h"
 #include "abl_runtime.hpp"
+#include "abl_pipeline.hpp"
 
 using namespace abl;
 
@@ -62,6 +63,13 @@ static const ProgramDesc *const *programs() {
       program_predator_prey(), nullptr,
   };
   return table;
+}
+
+const ProgramDesc *abl::find_program(const char *name) {
+  for (int i = 0; name && programs()[i]; i++) {
+    if (std::strcmp(programs()[i]->name, name) == 0) return programs()[i];
+  }
+  return nullptr;
 }
 
 extern "C" int abl_hip_program_count(void) {
@@ -312,99 +320,6 @@ __global__ void k_bin(const P *__restrict__ px, const P *__restrict__ py, const 
   uint32_t k = ((uint32_t) c.z * (uint32_t) g.n[1] + (uint32_t) c.y) * (uint32_t) g.n[0] + (uint32_t) c.x;
   key[i] = k;
   atomicAdd(&count[k], 1u);
-}
-
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanBlock * kScanItems;
-
-// Per tile: exclusive scan of count into start, tile total into bsum.
-// INV scans 1 - count (the survivors of a removeCurrent() flag array).
-template <typename T, bool INV>
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const T *__restrict__ count,
-                                                          uint32_t *__restrict__ start,
-                                                          uint32_t *__restrict__ bsum, int ncells) {
-  __shared__ uint32_t lds[kWave + 1];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    uint32_t c = base + k < ncells ? (uint32_t) count[base + k] : (INV ? 1u : 0u);
-    v[k] = INV ? 1u - c : c;
-    sum += v[k];
-  }
-  uint32_t total;
-  uint32_t pre = block_excl_scan(sum, lds, &total);
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    if (base + k < ncells) start[base + k] = pre;
-    pre += v[k];
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// Exclusive scan of the tile totals (one block, any number of tiles).
-__global__ __launch_bounds__(1024) void k_scan_totals(uint32_t *__restrict__ bsum, int ntiles,
-                                                     uint32_t *__restrict__ start, int ncells) {
-  __shared__ uint32_t lds[kWave + 1];
-  uint32_t carry = 0;
-  for (int base = 0; base < ntiles; base += blockDim.x) {
-    const int t = base + threadIdx.x;
-    uint32_t v = t < ntiles ? bsum[t] : 0u;
-    uint32_t total;
-    uint32_t pre = block_excl_scan(v, lds, &total);
-    if (t < ntiles) bsum[t] = pre + carry;
-    carry += total;
-  }
-  if (threadIdx.x == 0) start[ncells] = carry;
-}
-
-__global__ void k_scan_add(uint32_t *__restrict__ start, const uint32_t *__restrict__ bsum,
-                           int ncells) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncells) start[c] += bsum[c / kScanTile];
-}
-
-__global__ void k_scatter(const uint32_t *__restrict__ key, uint32_t *__restrict__ count,
-                          const uint32_t *__restrict__ start, const int32_t *__restrict__ id,
-                          uint32_t *__restrict__ tmp_idx, int32_t *__restrict__ tmp_id, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = key[i];
-  const uint32_t slot = start[k] + atomicSub(&count[k], 1u) - 1u;
-  tmp_idx[slot] = (uint32_t) i;
-  tmp_id[slot] = id[i];
-}
-
-struct GatherCols {
-  const void *src[kMaxCols];
-  void *dst[kMaxCols];
-  int elem[kMaxCols];
-  int ncols;
-};
-
-// Stable placement inside each cell by id, fused with the column gather.
-__global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const int32_t *__restrict__ tmp_id,
-                              const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
-                              GatherCols g, int32_t *__restrict__ id_out, int n) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t src = tmp_idx[s];
-  const int32_t my = tmp_id[s];
-  const uint32_t k = key[src];
-  const uint32_t a = start[k], b = start[k + 1];
-  uint32_t rank = 0;
-  for (uint32_t t = a; t < b; t++) rank += tmp_id[t] < my ? 1u : 0u;
-  const uint32_t dst = a + rank;
-  id_out[dst] = my;
-  for (int c = 0; c < g.ncols; c++) {
-    switch (g.elem[c]) {
-      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
-      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
-      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
-    }
-  }
 }
 
 // ---- kernels: run-time removal and birth -----------------------------------------

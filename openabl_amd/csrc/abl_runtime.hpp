@@ -43,6 +43,26 @@ struct ParamDesc {
 };
 
 struct StepLaunch;
+struct ColSet;
+
+// Hooks a single-type, single-step program provides so many independent runs
+// of it can be batched into one launch per kernel (abl_ensemble.hip).
+struct EnsembleHooks {
+  size_t (*args_size)(int precision);
+  // tile shape of variant `tile` (cells per tile in x, y, z)
+  void (*tile_shape)(int tile, int *tx, int *ty, int *tz);
+  // fill one member's kernel arguments (including its LDS capacity)
+  void (*fill_args)(int precision, void *dst, const double *params, const ColSet &in,
+                    const ColSet &out, const GridView &g, float near_t, float rc, int n,
+                    float occupancy, int tile);
+  // one launch over every member's tiles
+  void (*launch_multi)(int precision, const void *d_args, const uint32_t *d_tile_base,
+                       int members, unsigned total_tiles, int reach, float occupancy, int tile,
+                       int block, hipStream_t stream);
+  // main()'s agent records for these parameters (generator already seeded)
+  int (*host_records)(const double *params, int precision, std::vector<char> &out,
+                      size_t *count);
+};
 
 struct StepDesc {
   const char *name;
@@ -70,6 +90,8 @@ struct ProgramDesc {
   // sequential step (run on the host once per timestep after the parallel
   // steps; uses the reduction entry points), or nullptr
   int (*seq_step)(struct ::abl_sim *sim, const double *params);
+  // batched-ensemble support, or nullptr
+  const EnsembleHooks *ensemble;
 };
 
 // Column pointers of one agent type (members in declaration order, vector
@@ -137,6 +159,8 @@ const ProgramDesc *program_circle3d();
 const ProgramDesc *program_boids2d();
 const ProgramDesc *program_game_of_life();
 const ProgramDesc *program_predator_prey();
+// registered program by name, or nullptr (abl_runtime.hip)
+const ProgramDesc *find_program(const char *name);
 
 inline double param_of(const StepLaunch &L, int idx) { return L.params[idx]; }
 

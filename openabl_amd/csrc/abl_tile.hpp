@@ -180,17 +180,45 @@ namespace abl {
 // loop reads.  Candidates are visited row by row of the (narrowed) stencil,
 // each row in cell-sorted order: the same order as for_each_candidate.
 template <class M, int TX, int TY, int TZ, int RM>
+__device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned tile,
+                                           unsigned char *smem);
+
+template <class M, int TX, int TY, int TZ, int RM>
 __global__ __launch_bounds__(256) void tiled_step(typename M::Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  tiled_body<M, TX, TY, TZ, RM>(a, xcd_block(blockIdx.x, gridDim.x), smem);
+}
+
+// Ensemble form: the tiles of many independent members in one launch.
+// Member m owns tiles [tile_base[m], tile_base[m+1]) and has its own
+// arguments (grid, parameters); the block's member is uniform, so its
+// arguments are read through the scalar cache.
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_step_multi(const typename M::Args *__restrict__ args,
+                                                        const uint32_t *__restrict__ tile_base,
+                                                        int members) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  int lo = 0, hi = members - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_base[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  tiled_body<M, TX, TY, TZ, RM>(args[lo], t - tile_base[lo], smem);
+}
+
+template <class M, int TX, int TY, int TZ, int RM>
+__device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned tile,
+                                           unsigned char *smem) {
   using F = typename M::F;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
   using TT = TileTables<DIM, TX, TY, TZ, RM>;
-  extern __shared__ __align__(16) unsigned char smem[];
   TT &T = *reinterpret_cast<TT *>(smem);
   F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
   const GridView &g = a.g;
   const int R = g.reach;
-  T.setup(g, R, xcd_block(blockIdx.x, gridDim.x), a.cap);
+  T.setup(g, R, tile, a.cap);
   const bool ovf = T.overflow != 0;
   if (!ovf) {
     for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
@@ -247,10 +275,10 @@ __global__ __launch_bounds__(256) void tiled_step(typename M::Args a) {
   }
 }
 
-// Host side: LDS budget and launch of tiled_step for a tile shape.
+// Host side: LDS record capacity (twice the expected halo population, within
+// a 64 KiB block budget) and the block's dynamic LDS bytes.
 template <class M, int TX, int TY, int TZ, int RM>
-inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy, int block,
-                         hipStream_t stream) {
+inline void tile_budget(float occupancy, int *cap, size_t *smem) {
   using F = typename M::F;
   using TT = TileTables<M::DIM, TX, TY, TZ, RM>;
   const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
@@ -259,11 +287,25 @@ inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy,
   const size_t budget = 64 * 1024 - head;
   const int max_cap = (int) (budget / (M::NS * sizeof(F)));
   const int want = (int) (2.0 * halo_cells * (double) occupancy) + 128;
-  a.cap = want < max_cap ? want : max_cap;
-  const size_t smem = head + (size_t) a.cap * M::NS * sizeof(F);
-  unsigned ntiles = (unsigned) (((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY));
-  if (M::DIM == 3) ntiles *= (unsigned) ((g.n[2] + TZ - 1) / TZ);
+  *cap = want < max_cap ? want : max_cap;
+  *smem = head + (size_t) *cap * M::NS * sizeof(F);
+}
+
+template <int TX, int TY, int TZ, int DIM>
+inline unsigned tile_count(const GridView &g) {
+  unsigned n = (unsigned) (((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY));
+  if (DIM == 3) n *= (unsigned) ((g.n[2] + TZ - 1) / TZ);
+  return n;
+}
+
+// Launch tiled_step for a tile shape.
+template <class M, int TX, int TY, int TZ, int RM>
+inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy, int block,
+                         hipStream_t stream) {
+  size_t smem = 0;
+  tile_budget<M, TX, TY, TZ, RM>(occupancy, &a.cap, &smem);
   const int threads = block > 256 ? 256 : block;
+  const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
   hipLaunchKernelGGL((tiled_step<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a);
 }
 

@@ -14,6 +14,7 @@
 // Both distance comparisons are done on the float-rounded square sum against
 // host-computed thresholds (exact: sqrtf is monotone), so the loop takes no
 // square root; only |new_velocity| needs one.
+#include <cstring>
 #include <vector>
 
 #include "abl_runtime.hpp"
@@ -205,10 +206,11 @@ void launch_typed(const StepLaunch &L) {
 // members initialise in order (pos, then velocity); inside each float2 the
 // y draw comes first (gcc's right-to-left argument order).
 template <typename F>
-int init_typed(abl_sim *sim, const double *p) {
+void records_typed(const double *p, std::vector<char> &out, size_t *count) {
   const int n = (int) p[P_NUM_AGENTS];
   const F lo = (F) p[P_MIN_POS], hi = (F) p[P_MAX_POS];
-  std::vector<F> aos(4 * (size_t) (n > 0 ? n : 0));
+  out.assign(4 * sizeof(F) * (size_t) (n > 0 ? n : 0), 0);
+  F *aos = reinterpret_cast<F *>(out.data());
   for (int i = 0; i < n; i++) {
     const F py = host_random<F>(lo, hi);
     const F px = host_random<F>(lo, hi);
@@ -220,12 +222,130 @@ int init_typed(abl_sim *sim, const double *p) {
     r[2] = vx;
     r[3] = vy;
   }
-  return abl_hip_upload(sim, 0, aos.data(), (size_t) n);
+  *count = (size_t) (n > 0 ? n : 0);
+}
+
+int host_records(const double *p, int precision, std::vector<char> &out, size_t *count) {
+  if (precision == 0) records_typed<double>(p, out, count);
+  else records_typed<float>(p, out, count);
+  return ABL_OK;
 }
 
 int host_init(abl_sim *sim, const double *p, int precision) {
-  return precision == 0 ? init_typed<double>(sim, p) : init_typed<float>(sim, p);
+  std::vector<char> recs;
+  size_t n = 0;
+  host_records(p, precision, recs, &n);
+  return abl_hip_upload(sim, 0, recs.data(), n);
 }
+
+// ---- batched ensembles (abl_ensemble.hip) ------------------------------------
+
+template <typename F>
+BoidArgs<F> make_args(const double *p, const ColSet &in, const ColSet &out, const GridView &g,
+                      float near_t, float rc, int n) {
+  BoidArgs<F> a{};
+  a.px = (const F *) in.c[0];
+  a.py = (const F *) in.c[1];
+  a.vx = (const F *) in.c[2];
+  a.vy = (const F *) in.c[3];
+  a.opx = (F *) out.c[0];
+  a.opy = (F *) out.c[1];
+  a.ovx = (F *) out.c[2];
+  a.ovy = (F *) out.c[3];
+  a.g = g;
+  a.near_t = near_t;
+  a.sep_u = sqrtf_lt_threshold((double) (F) p[P_SEPARATION_RADIUS]);
+  a.rc = rc;
+  a.n = n;
+  a.time_scale = (F) p[P_TIME_SCALE];
+  a.global_scale = (F) p[P_GLOBAL_SCALE];
+  a.steer = (F) p[P_STEER_SCALE];
+  a.collision = (F) p[P_COLLISION_SCALE];
+  a.match = (F) p[P_MATCH_SCALE];
+  a.max_speed = (F) p[P_MAX_SPEED];
+  a.min_pos = (F) p[P_MIN_POS];
+  a.max_pos = (F) p[P_MAX_POS];
+  return a;
+}
+
+void ens_tile_shape(int tile, int *tx, int *ty, int *tz) {
+  *tz = 1;
+  *ty = 8;
+  *tx = tile == 2 ? 8 : tile == 3 ? 32 : 16;
+}
+
+template <typename F, int TX, int TY, int RM>
+void ens_cap(float occupancy, int *cap, size_t *smem) {
+  tile_budget<BoidPolicy<F>, TX, TY, 1, RM>(occupancy, cap, smem);
+}
+
+template <typename F, int TX, int TY>
+void ens_cap_r(int reach, float occupancy, int *cap, size_t *smem) {
+  if (reach <= 1) ens_cap<F, TX, TY, 1>(occupancy, cap, smem);
+  else if (reach == 2) ens_cap<F, TX, TY, 2>(occupancy, cap, smem);
+  else ens_cap<F, TX, TY, 4>(occupancy, cap, smem);
+}
+
+template <typename F>
+void ens_cap_t(int tile, int reach, float occupancy, int *cap, size_t *smem) {
+  if (tile == 2) ens_cap_r<F, 8, 8>(reach, occupancy, cap, smem);
+  else if (tile == 3) ens_cap_r<F, 32, 8>(reach, occupancy, cap, smem);
+  else ens_cap_r<F, 16, 8>(reach, occupancy, cap, smem);
+}
+
+size_t ens_args_size(int precision) {
+  return precision == 0 ? sizeof(BoidArgs<double>) : sizeof(BoidArgs<float>);
+}
+
+void ens_fill_args(int precision, void *dst, const double *p, const ColSet &in, const ColSet &out,
+                   const GridView &g, float near_t, float rc, int n, float occupancy, int tile) {
+  size_t smem = 0;
+  if (precision == 0) {
+    BoidArgs<double> a = make_args<double>(p, in, out, g, near_t, rc, n);
+    ens_cap_t<double>(tile, g.reach, occupancy, &a.cap, &smem);
+    std::memcpy(dst, &a, sizeof a);
+  } else {
+    BoidArgs<float> a = make_args<float>(p, in, out, g, near_t, rc, n);
+    ens_cap_t<float>(tile, g.reach, occupancy, &a.cap, &smem);
+    std::memcpy(dst, &a, sizeof a);
+  }
+}
+
+template <typename F, int TX, int TY, int RM>
+void ens_launch(const void *args, const uint32_t *tile_base, int members, unsigned total,
+                float occupancy, int block, hipStream_t st) {
+  int cap = 0;
+  size_t smem = 0;
+  ens_cap<F, TX, TY, RM>(occupancy, &cap, &smem);
+  const int threads = block > 256 ? 256 : block;
+  hipLaunchKernelGGL((tiled_step_multi<BoidPolicy<F>, TX, TY, 1, RM>), total, threads, smem, st,
+                     (const BoidArgs<F> *) args, tile_base, members);
+}
+
+template <typename F, int TX, int TY>
+void ens_launch_r(const void *args, const uint32_t *tb, int m, unsigned total, int reach,
+                  float occ, int block, hipStream_t st) {
+  if (reach <= 1) ens_launch<F, TX, TY, 1>(args, tb, m, total, occ, block, st);
+  else if (reach == 2) ens_launch<F, TX, TY, 2>(args, tb, m, total, occ, block, st);
+  else ens_launch<F, TX, TY, 4>(args, tb, m, total, occ, block, st);
+}
+
+template <typename F>
+void ens_launch_t(const void *args, const uint32_t *tb, int m, unsigned total, int reach,
+                  float occ, int tile, int block, hipStream_t st) {
+  if (tile == 2) ens_launch_r<F, 8, 8>(args, tb, m, total, reach, occ, block, st);
+  else if (tile == 3) ens_launch_r<F, 32, 8>(args, tb, m, total, reach, occ, block, st);
+  else ens_launch_r<F, 16, 8>(args, tb, m, total, reach, occ, block, st);
+}
+
+void ens_launch_multi(int precision, const void *args, const uint32_t *tb, int m, unsigned total,
+                      int reach, float occ, int tile, int block, hipStream_t st) {
+  if (precision == 0) ens_launch_t<double>(args, tb, m, total, reach, occ, tile, block, st);
+  else ens_launch_t<float>(args, tb, m, total, reach, occ, tile, block, st);
+}
+
+const EnsembleHooks ensemble_hooks = {ens_args_size, ens_tile_shape, ens_fill_args,
+                                      ens_launch_multi, host_records};
 
 void launch_update_boid(const StepLaunch &L) {
   if (L.precision == 0) launch_typed<double>(L);
@@ -252,6 +372,8 @@ const ProgramDesc *program_boids2d() {
       {{"update_boid", 0, 0, "interaction_radius", 1.0, 0xfu, launch_update_boid}},
       env,
       host_init,
+      nullptr,
+      &ensemble_hooks,
   };
   return &d;
 }

@@ -75,6 +75,18 @@ _SIGNATURES = {
     "abl_server_run_times": ([C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
     "abl_server_stats": ([C.c_void_p, C.c_void_p], C.c_int),
     "abl_server_destroy": ([C.c_void_p], C.c_int),
+    "abl_ens_last_error": ([], C.c_char_p),
+    "abl_ens_member_seed": ([C.c_uint64, C.c_uint64, C.c_int, C.c_void_p], C.c_int),
+    "abl_ens_create": ([C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                        C.c_void_p], C.c_int),
+    "abl_ens_set_config": ([C.c_void_p, C.c_char_p, C.c_char_p], C.c_int),
+    "abl_ens_init": ([C.c_void_p, C.c_uint64, C.c_uint64], C.c_int),
+    "abl_ens_simulate": ([C.c_void_p, C.c_int], C.c_int),
+    "abl_ens_synchronize": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_ens_member_count": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "abl_ens_agents": ([C.c_void_p, C.c_void_p], C.c_int),
+    "abl_ens_download": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_ens_destroy": ([C.c_void_p], C.c_int),
 }
 
 
