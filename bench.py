@@ -39,11 +39,37 @@ PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.
               max_speed=2.0, time_scale=0.0005, global_scale=0.15)
 
 
+# launch configuration of the runs (hip.cell_scale: cells per granularity;
+# hip.tile: LDS tile shape of the neighbour kernel)
+CONFIG = {"hip.cell_scale": 1, "hip.tile": 1}
+
+
 def neighbour_bytes(n: int, precision: str, cells: int) -> int:
     """Algorithmic HBM bytes of one update_boid launch: every agent's pos and
-    velocity read once and written once, plus the cell offsets."""
-    fb = 8 if precision == "f64" else 4
-    return n * 4 * fb * 2 + 4 * (cells + 1)
+    velocity read once and written once, plus the cell offsets.  Per agent:
+    FP64 2 x 32 B, FP32 2 x 16 B, FP32-compensated 2 x (16 + 8) B."""
+    per_agent = {"f64": 64, "f32": 32, "f32c": 48}[precision]
+    return n * per_agent + 4 * (cells + 1)
+
+
+def kernel_name(precision: str) -> str:
+    """The neighbour kernel the runtime launches for CONFIG."""
+    tile, cs = CONFIG["hip.tile"], CONFIG["hip.cell_scale"]
+    if tile == 0:
+        return {"f64": "boids_step<double,false>", "f32": "boids_step<float,false>",
+                "f32c": "boids_step_c<false>"}[precision]
+    pol = {"f64": "BoidPolicy<double>", "f32": "BoidPolicy<float>", "f32c": "BoidPolicyC"}[precision]
+    tx = {1: 16, 2: 8, 3: 32}[tile]
+    rm = 1 if cs <= 1 else (2 if cs == 2 else 4)
+    return f"tiled_step<{pol}, {tx}, 8, 1, {rm}>"
+
+
+def grid_cells(box: float, granularity: float, cell_scale: int) -> int:
+    """Cells of the 2-D grid the runtime builds (cell edge granularity /
+    cell_scale, widened by 2^-20)."""
+    import math
+    cell = granularity / cell_scale * (1.0 + 1.0 / 1048576.0)
+    return max(1, math.ceil(box / cell)) ** 2
 
 
 def run_gpu(precision: str, steps: int, warmup: int, timing_steps: int):
@@ -51,7 +77,7 @@ def run_gpu(precision: str, steps: int, warmup: int, timing_steps: int):
     from openabl_amd import Simulation, load_library
     lib = load_library()
     lib.abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
-    sim = Simulation("boids2d", precision, params=PARAMS)
+    sim = Simulation("boids2d", precision, params=PARAMS, config=CONFIG)
     sim.init_agents()
     sim.simulate(warmup)
     sim.synchronize()
@@ -64,8 +90,8 @@ def run_gpu(precision: str, steps: int, warmup: int, timing_steps: int):
     sim.set_config("hip.timing", 1)
     sim.simulate(timing_steps)
     total_ms, nbr_ms, grid_ms = sim.last_timing()
-    box = sim.get_param("max_pos")
-    cells = int(-(-box // (10.0 * (1 + 1 / 1048576)))) ** 2
+    cells = grid_cells(sim.get_param("max_pos"), PARAMS["interaction_radius"],
+                       CONFIG["hip.cell_scale"])
     sim.close()
     return {
         "wall_s": wall,
@@ -99,6 +125,180 @@ def cpu_baseline(sample_agents: int):
             "seconds": secs}
 
 
+def ensemble_members(k=256, seed=2026):
+    """The batched-ensemble workload: boids2d members with agent counts
+    log-uniform in [1k, 64k], box side sqrt(N/0.065), weights U[0.5, 2]."""
+    import math
+    import numpy as np
+    from openabl_amd import fold6
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in np.exp(rng.uniform(math.log(1000), math.log(65536), k)).astype(int):
+        w = rng.uniform(0.5, 2.0, 3)
+        out.append({"num_agents": int(n), "agent_density": 0.065,
+                    "max_pos": fold6(math.sqrt(n / 0.065)), "steer_scale": float(w[0]),
+                    "collision_scale": float(w[1]), "match_scale": float(w[2])})
+    return out
+
+
+def run_ensemble(args):
+    """256 uneven boids2d members, one launch per kernel, graph-replayed
+    timesteps; beside it the same members run alone one after another (a
+    sample of them, extrapolated) on the plain per-run path."""
+    from openabl_amd import Simulation, load_library
+    from openabl_amd.ensemble import Ensemble, member_seed
+    ms = ensemble_members()
+    total = sum(m["num_agents"] for m in ms)
+    res = {}
+    for prec in ("f64", "f32"):
+        ens = Ensemble("boids2d", prec, ms, config={"hip.cell_scale": 2, "hip.tile": 1})
+        ens.init(0xDEADBEEF, 0xBEEFDEAD)
+        ens.simulate(args.warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.simulate(args.ens_steps)
+        dev_ms = ens.synchronize()
+        wall = time.perf_counter() - t0
+        ens.close()
+        # plain path: members alone, sequentially (every 8th member, scaled)
+        sample = ms[::8]
+        t_alone = 0.0
+        for k, p in enumerate(sample):
+            load_library().abl_rng_seed(*member_seed(0xDEADBEEF, 0xBEEFDEAD, 8 * k))
+            sim = Simulation("boids2d", prec, params=p, config={"hip.cell_scale": 2, "hip.tile": 1})
+            sim.init_agents()
+            sim.simulate(5)
+            sim.synchronize()
+            t1 = time.perf_counter()
+            sim.simulate(args.alone_steps)
+            sim.synchronize()
+            t_alone += (time.perf_counter() - t1) / args.alone_steps
+            sim.close()
+        sample_agents = sum(p["num_agents"] for p in sample)
+        alone_rate = sample_agents / t_alone
+        res[prec] = {"agent_steps_per_s": round(total * args.ens_steps / wall, 1),
+                     "device_ms_per_step": round(dev_ms / args.ens_steps, 5),
+                     "alone_sequential_agent_steps_per_s": round(alone_rate, 1),
+                     "speedup_vs_alone": round(total * args.ens_steps / wall / alone_rate, 2)}
+    print(json.dumps({"metric": "batched ensemble agent-steps/s (256 boids2d members)",
+                      "members": len(ms), "agents": total, "steps": args.ens_steps,
+                      "config": "N log-uniform [1k, 64k], box sqrt(N/0.065), weights U[0.5,2], "
+                                "cell_scale 2, 16x8 tiles, graph-replayed timesteps",
+                      **res}))
+
+
+CIRCLE3D_PARAMS = dict(num_agents=1 << 20, rho=(1 << 20) / 1e9, r=4.0, k_rep=0.05, k_att=0.01,
+                       num_clusters=64, cluster_sigma=20.0, background_fraction=0.2)
+
+
+def run_precision(args):
+    """circle3d, 1,048,576 agents in a 1000^3 box from 64 Gaussian clusters
+    (sigma 20) plus 20% uniform background, radius 8: agent-steps/s of the
+    FP64 path and of FP32-compensated, and the f32c error against the FP64
+    path per step over 100 steps from the same initial state."""
+    import numpy as np
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    res = {}
+    for prec in ("f64", "f32c"):
+        load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+        sim = Simulation("circle3d", prec, params=CIRCLE3D_PARAMS)
+        sim.init_agents()
+        sim.simulate(args.warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.simulate(args.prec_steps)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        sim.set_config("hip.timing", 1)
+        sim.simulate(10)
+        _, nbr_ms, grid_ms = sim.last_timing()
+        sim.close()
+        res[prec] = {"agent_steps_per_s": round(N_AGENTS * args.prec_steps / wall, 1),
+                     "ms_per_step": round(1e3 * wall / args.prec_steps, 5),
+                     "nbr_kernel_ms": round(nbr_ms / 10, 5), "grid_ms": round(grid_ms / 10, 5)}
+    # error: both precisions from the same initial state, compared every step
+    dt = record_dtype(MEMBERS["circle3d"], "f64")
+    sims = {}
+    for prec in ("f64", "f32c"):
+        load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+        sims[prec] = Simulation("circle3d", prec, params=CIRCLE3D_PARAMS)
+        sims[prec].init_agents()
+    abs_err, rel_err = [], []
+    for t in range(args.prec_steps):
+        for s in sims.values():
+            s.simulate(1)
+        a = sims["f64"].download(dt)["pos"]
+        b = sims["f32c"].download(dt)["pos"]
+        d = np.abs(a - b)
+        abs_err.append(float(d.max()))
+        rel_err.append(float((d / np.maximum(np.abs(a), 1.0)).max()))
+    for s in sims.values():
+        s.close()
+    print(json.dumps({
+        "metric": "agent-steps/s, 1M circle3d clustered, FP32-compensated vs FP64",
+        "value": res["f32c"]["agent_steps_per_s"], "unit": "agent-steps/s",
+        "f64": res["f64"], "f32c": res["f32c"],
+        "speedup": round(res["f32c"]["agent_steps_per_s"] / res["f64"]["agent_steps_per_s"], 3),
+        "error_vs_f64": {"steps": args.prec_steps,
+                         "max_abs_per_step": [float(f"{e:.3e}") for e in abs_err],
+                         "max_rel_per_step": [float(f"{e:.3e}") for e in rel_err]},
+        "config": {"workload": "circle3d_clustered 1,048,576 agents, 1000^3 box, 64 clusters "
+                               "sigma 20 + 20% uniform, radius 8", "steps": args.prec_steps},
+    }))
+
+
+def run_serving(args):
+    """Poisson arrivals of circle (FP64), boids2d (FP32), game_of_life and
+    predator_prey runs, agent counts log-uniform 1k-1M, 100-1000 steps,
+    admitted by HBM footprint; p50/p99 latency beside aggregate agent-steps/s."""
+    import math
+    import numpy as np
+    from openabl_amd.serve import Server
+    rng = np.random.default_rng(7)
+    kinds = [("circle", "f64"), ("boids2d", "f32"), ("game_of_life", "f64"),
+             ("predator_prey", "f64")]
+    reqs = []
+    t = 0.0
+    for k in range(args.serve_runs):
+        prog, prec = kinds[k % len(kinds)]
+        n = int(np.exp(rng.uniform(math.log(1000), math.log(args.serve_max_agents))))
+        steps = int(rng.integers(100, 1001))
+        if prog == "circle":
+            p = {"num_agents": n, "rho": 0.1024, "r": 2.5}
+        elif prog == "boids2d":
+            p = {"num_agents": n, "agent_density": 0.065536}
+        elif prog == "game_of_life":
+            p = {"num_agents": n, "alive_fraction": 0.3}
+        else:
+            p = {"num_agents": n, "prey_fraction": 0.4, "predator_fraction": 0.04}
+        t += rng.exponential(args.serve_interval)
+        reqs.append((t, prog, prec, p, steps))
+    srv = Server(hbm_budget=int(args.serve_budget_gb * (1 << 30)), max_active=args.serve_active,
+                 chunk_steps=25)
+    t0 = time.perf_counter()
+    ids = []
+    for (at, prog, prec, p, steps) in reqs:
+        while time.perf_counter() - t0 < at:
+            time.sleep(0.0005)
+        ids.append(srv.submit(prog, prec, p, steps))
+    for rid in ids:
+        srv.wait(rid)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    print(json.dumps({"metric": "serving: p50/p99 run latency and agent-steps/s under load",
+                      "runs": len(reqs), "p50_s": round(st["p50_s"], 4),
+                      "p99_s": round(st["p99_s"], 4),
+                      "agent_steps_per_s": round(st["agent_steps"] / wall, 1),
+                      "wall_s": round(wall, 3), "peak_admitted_bytes": st["peak_bytes"],
+                      "config": {"mix": "circle f64, boids2d f32, game_of_life, predator_prey",
+                                 "agents": f"log-uniform 1k-{args.serve_max_agents}",
+                                 "steps": "uniform 100-1000",
+                                 "mean_interarrival_s": args.serve_interval,
+                                 "budget_gb": args.serve_budget_gb,
+                                 "max_active": args.serve_active}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,20 +308,40 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--mode", choices=["main", "ensemble", "serving", "precision"], default="main")
+    ap.add_argument("--prec-steps", type=int, default=100)
+    ap.add_argument("--ens-steps", type=int, default=500)
+    ap.add_argument("--alone-steps", type=int, default=50)
+    ap.add_argument("--serve-runs", type=int, default=40)
+    ap.add_argument("--serve-max-agents", type=int, default=1 << 20)
+    ap.add_argument("--serve-interval", type=float, default=0.05)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-active", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the 1M-agent workload fits one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "ensemble":
+        run_ensemble(args)
+        return
+    if args.mode == "serving":
+        run_serving(args)
+        return
+    if args.mode == "precision":
+        run_precision(args)
+        return
 
     f64 = run_gpu("f64", args.steps, args.warmup, args.timing_steps)
     f32 = None if args.no_fp32 else run_gpu("f32", args.steps, args.warmup, args.timing_steps)
+    f32c = None if args.no_fp32 else run_gpu("f32c", args.steps, args.warmup, args.timing_steps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample)
 
     b = neighbour_bytes(N_AGENTS, "f64", f64["cells"])
     achieved = b / (f64["nbr_ms_avg"] * 1e-3) / 1e9
     line = {
         "metric": "agent-steps/sec, 1M boids2d FP64 vs FP32",
+        "launch_config": CONFIG,
         "value": round(f64["agent_steps_per_s"], 1),
         "unit": "agent-steps/s",
         "n_gpus": 1,
@@ -139,18 +359,19 @@ def main():
                    "step": "grid rebuild + update_boid"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "boids_step<double,false>",
+                     "kernel": kernel_name("f64"),
                      "bytes_per_launch": b, "avg_kernel_ms": round(f64["nbr_ms_avg"], 5)},
         "grid_ms_per_step": round(f64["grid_ms_avg"], 5),
         "cpu_baseline": cpu and {k: v for k, v in cpu.items() if k != "seconds"},
     }
-    if f32:
-        b32 = neighbour_bytes(N_AGENTS, "f32", f32["cells"])
-        line["fp32"] = {"value": round(f32["agent_steps_per_s"], 1),
-                        "ms_per_step": round(f32["ms_per_step"], 5),
-                        "avg_kernel_ms": round(f32["nbr_ms_avg"], 5),
-                        "grid_ms_per_step": round(f32["grid_ms_avg"], 5),
-                        "kernel_GBs": round(b32 / (f32["nbr_ms_avg"] * 1e-3) / 1e9, 1)}
+    for key, prec, r in (("fp32", "f32", f32), ("fp32_compensated", "f32c", f32c)):
+        if r:
+            bp = neighbour_bytes(N_AGENTS, prec, r["cells"])
+            line[key] = {"value": round(r["agent_steps_per_s"], 1),
+                         "ms_per_step": round(r["ms_per_step"], 5),
+                         "avg_kernel_ms": round(r["nbr_ms_avg"], 5),
+                         "grid_ms_per_step": round(r["grid_ms_avg"], 5),
+                         "kernel_GBs": round(bp / (r["nbr_ms_avg"] * 1e-3) / 1e9, 1)}
     print(json.dumps(line))
 
 

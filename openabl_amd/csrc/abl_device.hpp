@@ -65,6 +65,23 @@ __device__ __forceinline__ float dist_sq_f32(F ax, F ay, F az, F bx, F by, F bz)
   return (float) s;
 }
 
+// Compensated (Kahan) float sum for the FP32-compensated precision: the
+// running error c is fed back into the next addend.  Builds use
+// -ffp-contract=off and no reassociation, so the compensation survives.
+struct KahanF {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double) s - (double) c; }
+};
+
+// Fast float reciprocal (1 ulp) for the FP32-compensated precision.
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Inclusive-sum scan across one wave64.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int lane = threadIdx.x & (kWave - 1);

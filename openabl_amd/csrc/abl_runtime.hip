@@ -90,8 +90,10 @@ namespace {
 struct Column {
   int member;
   int comp;
-  int elem;          // bytes per value: 1 (bool), 4 (int, float), 8 (double)
+  int elem;          // bytes per value on the device: 1 (bool), 4 (int, float), 8 (double)
   size_t aos_off;    // byte offset in the generated struct
+  int aos_elem;      // bytes per value in the generated struct (8 with elem 4: a double
+                     // held as float on the device, ABL_PREC_F32C)
 };
 
 struct AgentStore {
@@ -161,23 +163,29 @@ struct abl_sim {
 // C struct layout of the generated agent struct (src/backend/CPrinter.cpp:250-253):
 // members in order, each aligned to its scalar size, total padded to the
 // largest alignment.
+// ABL_PREC_F32C keeps the FP64 program's struct (host initialisation and
+// save() unchanged) and holds, on the device, positions as double and every
+// other float member as float.
 static void layout_agent(AgentStore &s, const AgentDesc &d, int precision) {
   s.desc = &d;
   s.cols.clear();
   size_t off = 0, align_max = 1;
-  const int fb = precision == ABL_PREC_F64 ? 8 : 4;
+  const int fb = precision == ABL_PREC_F32 ? 4 : 8;
   for (size_t m = 0; m < d.members.size(); m++) {
     const MemberDesc &md = d.members[m];
-    int elem = md.kind == K_BOOL ? 1 : md.kind == K_INT ? 4 : fb;
-    size_t a = (size_t) elem;
+    const int aelem = md.kind == K_BOOL ? 1 : md.kind == K_INT ? 4 : fb;
+    const int elem = (precision == ABL_PREC_F32C && aelem == 8 && !md.is_pos) ? 4 : aelem;
+    size_t a = (size_t) aelem;
     off = (off + a - 1) / a * a;
     int nc = kind_components(md.kind);
     if (md.is_pos) {
       s.pos_col = (int) s.cols.size();
       s.dim = nc;
     }
-    for (int c = 0; c < nc; c++) s.cols.push_back(Column{(int) m, c, elem, off + (size_t) c * elem});
-    off += (size_t) nc * elem;
+    for (int c = 0; c < nc; c++) {
+      s.cols.push_back(Column{(int) m, c, elem, off + (size_t) c * aelem, aelem});
+    }
+    off += (size_t) nc * aelem;
     align_max = std::max(align_max, a);
   }
   s.aos_size = (off + align_max - 1) / align_max * align_max;
@@ -272,6 +280,7 @@ struct ColXfer {
   void *col[kMaxCols];
   uint32_t aos_off[kMaxCols];
   int elem[kMaxCols];
+  int narrow[kMaxCols];  // 1: double in the struct, float on the device
   int ncols;
   uint32_t aos_size;
 };
@@ -283,6 +292,10 @@ __global__ void k_aos_to_soa(const char *__restrict__ aos, ColXfer x, int32_t *_
   const char *rec = aos + (size_t) i * x.aos_size;
   for (int c = 0; c < x.ncols; c++) {
     const char *p = rec + x.aos_off[c];
+    if (x.narrow[c]) {
+      ((float *) x.col[c])[i] = (float) *(const double *) p;
+      continue;
+    }
     switch (x.elem[c]) {
       case 1: ((uint8_t *) x.col[c])[i] = *(const uint8_t *) p; break;
       case 4: ((uint32_t *) x.col[c])[i] = *(const uint32_t *) p; break;
@@ -301,6 +314,10 @@ __global__ void k_soa_to_aos(char *__restrict__ aos, ColXfer x, const int32_t *_
   char *rec = aos + (size_t) (id ? id[s] : s) * x.aos_size;
   for (int c = 0; c < x.ncols; c++) {
     char *p = rec + x.aos_off[c];
+    if (x.narrow[c]) {
+      *(double *) p = (double) ((const float *) x.col[c])[s];
+      continue;
+    }
     switch (x.elem[c]) {
       case 1: *(uint8_t *) p = ((const uint8_t *) x.col[c])[s]; break;
       case 4: *(uint32_t *) p = ((const uint32_t *) x.col[c])[s]; break;
@@ -566,7 +583,7 @@ static double step_radius(const abl_sim *sim, const StepDesc &st) {
       // under use_float the parameter is a float global, widened for the
       // double-literal product the near radius expression prints as
       double v = sim->params[p];
-      if (sim->precision != ABL_PREC_F64) v = (double) (float) v;
+      if (sim->precision == ABL_PREC_F32) v = (double) (float) v;
       return st.radius_scale * v;
     }
   }
@@ -581,13 +598,12 @@ static double step_radius(const abl_sim *sim, const StepDesc &st) {
 extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out) {
   if (!program || !out) return fail(ABL_ERR_ARG, "null argument");
   *out = nullptr;
-  if (precision != ABL_PREC_F64 && precision != ABL_PREC_F32)
-    return fail(ABL_ERR_UNSUPPORTED, "precision not supported by this build");
-  const ProgramDesc *prog = nullptr;
-  for (int i = 0; programs()[i]; i++) {
-    if (std::strcmp(programs()[i]->name, program) == 0) prog = programs()[i];
-  }
+  if (precision != ABL_PREC_F64 && precision != ABL_PREC_F32 && precision != ABL_PREC_F32C)
+    return fail(ABL_ERR_UNSUPPORTED, "unknown precision");
+  const ProgramDesc *prog = find_program(program);
   if (!prog) return fail(ABL_ERR_ARG, std::string("unknown program ") + program);
+  if (precision == ABL_PREC_F32C && !prog->f32c)
+    return fail(ABL_ERR_UNSUPPORTED, std::string(program) + " has no FP32-compensated form");
   abl_sim *sim = new abl_sim();
   sim->prog = prog;
   sim->precision = precision;
@@ -712,6 +728,7 @@ static ColXfer xfer_of(const AgentStore &s, bool current) {
     x.col[c] = current ? col_cur(s, c) : col_other(s, c);
     x.aos_off[c] = (uint32_t) s.cols[c].aos_off;
     x.elem[c] = s.cols[c].elem;
+    x.narrow[c] = s.cols[c].aos_elem != s.cols[c].elem ? 1 : 0;
   }
   return x;
 }

@@ -92,6 +92,8 @@ struct ProgramDesc {
   int (*seq_step)(struct ::abl_sim *sim, const double *params);
   // batched-ensemble support, or nullptr
   const EnsembleHooks *ensemble;
+  // has an ABL_PREC_F32C form (FP32 neighbour arithmetic, FP64 positions)
+  bool f32c;
 };
 
 // Column pointers of one agent type (members in declaration order, vector

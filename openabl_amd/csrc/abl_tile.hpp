@@ -179,6 +179,22 @@ namespace abl {
 // Records are the first DIM values = position, then the members the near
 // loop reads.  Candidates are visited row by row of the (narrowed) stencil,
 // each row in cell-sorted order: the same order as for_each_candidate.
+//
+// A policy that declares `static constexpr bool kRelative = true` keeps FP64
+// positions in HBM but stages them in LDS as float offsets from the tile's
+// centre (FP32-compensated precision).  It then supplies instead
+//   static void load(const Args &, uint32_t slot, F *rec, const double org[3])
+//   static void position(const Args &, uint32_t slot, double p[3])  -- FP64 position
+//   static void finish(const Args &, Acc &, const F *own, uint32_t slot, const double p[3])
+template <class M, class = void>
+struct policy_relative {
+  static constexpr bool value = false;
+};
+template <class M>
+struct policy_relative<M, decltype((void) M::kRelative)> {
+  static constexpr bool value = M::kRelative;
+};
+
 template <class M, int TX, int TY, int TZ, int RM>
 __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned tile,
                                            unsigned char *smem);
@@ -213,6 +229,7 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   using F = typename M::F;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
+  constexpr bool REL = policy_relative<M>::value;
   using TT = TileTables<DIM, TX, TY, TZ, RM>;
   TT &T = *reinterpret_cast<TT *>(smem);
   F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
@@ -220,9 +237,20 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   const int R = g.reach;
   T.setup(g, R, tile, a.cap);
   const bool ovf = T.overflow != 0;
+  // origin of relative records: the centre of the tile's own cells
+  double org[3] = {0.0, 0.0, 0.0};
+  if constexpr (REL) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+      org[d] = g.lo[d] + ((double) T.t0[d] + 0.5 * (double) T.tn[d]) / g.inv_cell;
+  }
+  auto load = [&](uint32_t j, F *r) {
+    if constexpr (REL) M::load(a, j, r, org);
+    else M::load(a, j, r);
+  };
   if (!ovf) {
     for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
-      M::load(a, T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k), rec + (size_t) NS * k);
+      load(T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k), rec + (size_t) NS * k);
     }
   }
   __syncthreads();
@@ -236,12 +264,20 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
 #pragma unroll
       for (int v = 0; v < NS; v++) own[v] = rec[(size_t) NS * ol + v];
     } else {
-      M::load(a, gi, own);
+      load(gi, own);
     }
-    const int3 c = cell_of<F>(g, own[0], own[1], DIM == 3 ? own[2] : (F) 0);
-    const float ux = (float) (((double) own[0] - g.lo[0]) * g.inv_cell - (double) c.x);
-    const float uy = (float) (((double) own[1] - g.lo[1]) * g.inv_cell - (double) c.y);
-    const float uz = DIM == 3 ? (float) (((double) own[2] - g.lo[2]) * g.inv_cell - (double) c.z) : 0.f;
+    // the agent's cell comes from its stored position, exactly as binned
+    double pd[3] = {0.0, 0.0, 0.0};
+    if constexpr (REL) {
+      M::position(a, gi, pd);
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
+    }
+    const int3 c = cell_of<double>(g, pd[0], pd[1], pd[2]);
+    const float ux = (float) ((pd[0] - g.lo[0]) * g.inv_cell - (double) c.x);
+    const float uy = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) c.y);
+    const float uz = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) c.z) : 0.f;
     typename M::Acc acc;
     M::begin(a, acc, own);
     const int zr = DIM == 3 ? R : 0;
@@ -265,13 +301,14 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
           const uint32_t g0 = T.global_of(r, l0);
           for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
             F nb[NS];
-            M::load(a, j, nb);
+            load(j, nb);
             acc.visit(a, own, nb);
           }
         }
       }
     }
-    M::finish(a, acc, own, gi);
+    if constexpr (REL) M::finish(a, acc, own, gi, pd);
+    else M::finish(a, acc, own, gi);
   }
 }
 

@@ -154,12 +154,170 @@ struct BoidPolicy {
   }
 };
 
-template <typename F, int TX, int TY>
-void launch_tile_r(const StepLaunch &L, const BoidArgs<F> &a) {
-  using M = BoidPolicy<F>;
+// ---- FP32-compensated precision (ABL_PREC_F32C) -----------------------------
+// Positions stay FP64 in HBM and are integrated in FP64; velocity is stored
+// as float.  The near loop runs in float on offsets nx.pos - in.pos
+// (|offset| <= interaction_radius): LDS records hold positions relative to
+// the tile centre, and the loop accumulates the accepted offsets and
+// neighbour velocities in Kahan-compensated float sums.  Since
+// (sum of nx.pos) / count - pos = (sum of offsets) / count, the steer and
+// avoid terms come straight from those sums.  The near test is the float
+// square sum of the offset against the same thresholds, so it can differ
+// from the FP64 test only for pairs within float rounding of the radius.
+struct BoidArgsC {
+  const double *px, *py;
+  const float *vx, *vy;
+  double *opx, *opy;
+  float *ovx, *ovy;
+  const int32_t *id;
+  GridView g;
+  float near_t, sep_u, rc;
+  int cap, n;
+  float global_scale, steer, collision, match, max_speed;
+  double time_scale, min_pos, max_pos;
+};
+
+struct BoidAccC {
+  KahanF dx, dy, gvx, gvy, cx, cy;
+  int icount = 0, ccount = 0;
+  __device__ __forceinline__ void visit(const BoidArgsC &a, float ox, float oy, float qvx,
+                                        float qvy) {
+    const float f = ox * ox + oy * oy;
+    if (f > a.near_t) return;
+    dx.add(ox);
+    dy.add(oy);
+    gvx.add(qvx);
+    gvy.add(qvy);
+    icount += 1;
+    if (f < a.sep_u) {
+      cx.add(ox);
+      cy.add(oy);
+      ccount += 1;
+    }
+  }
+};
+
+__device__ __forceinline__ void boid_finish_c(const BoidArgsC &a, const BoidAccC &s, double x,
+                                              double y, float vx, float vy, uint32_t i) {
+  float chx = 0.f, chy = 0.f;
+  if (s.icount > 0) {
+    const float n = (float) s.icount;
+    chx = chx + ((float) s.dx.value() / n) * a.steer;   // steer: mean - pos
+    chy = chy + ((float) s.dy.value() / n) * a.steer;
+    chx = chx + ((float) s.gvx.value() / n) * a.match;  // match: mean velocity
+    chy = chy + ((float) s.gvy.value() / n) * a.match;
+  }
+  if (s.ccount > 0) {
+    const float n = (float) s.ccount;
+    chx = chx - ((float) s.cx.value() / n) * a.collision;  // avoid: pos - mean
+    chy = chy - ((float) s.cy.value() / n) * a.collision;
+  }
+  float nvx = vx + chx * a.global_scale;
+  float nvy = vy + chy * a.global_scale;
+  const float speed = __builtin_sqrtf(nvx * nvx + nvy * nvy);
+  if (speed > a.max_speed) {
+    nvx = (nvx / speed) * a.max_speed;
+    nvy = (nvy / speed) * a.max_speed;
+  }
+  double npx = x + (double) nvx * a.time_scale;
+  double npy = y + (double) nvy * a.time_scale;
+  npx = npx < a.min_pos ? a.max_pos : (npx > a.max_pos ? a.min_pos : npx);
+  npy = npy < a.min_pos ? a.max_pos : (npy > a.max_pos ? a.min_pos : npy);
+  a.opx[i] = npx;
+  a.opy[i] = npy;
+  a.ovx[i] = nvx;
+  a.ovy[i] = nvy;
+}
+
+// Untiled FP32-compensated form: offsets from the FP64 difference.
+template <bool BY_ID>
+__global__ __launch_bounds__(1024) void boids_step_c(BoidArgsC a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  const double x = a.px[i], y = a.py[i];
+  const int3 c = cell_of<double>(a.g, x, y, 0.0);
+  BoidAccC s;
+  auto body = [&](uint32_t j) {
+    s.visit(a, (float) (a.px[j] - x), (float) (a.py[j] - y), a.vx[j], a.vy[j]);
+  };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, 0, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, 0, a.g.reach, body);
+  }
+  boid_finish_c(a, s, x, y, a.vx[i], a.vy[i], (uint32_t) i);
+}
+
+struct BoidPolicyC {
+  using F = float;
+  static constexpr int DIM = 2, NS = 4;
+  static constexpr bool kRelative = true;
+  using Args = BoidArgsC;
+  __device__ static void load(const Args &a, uint32_t j, float *r, const double *org) {
+    r[0] = (float) (a.px[j] - org[0]);
+    r[1] = (float) (a.py[j] - org[1]);
+    r[2] = a.vx[j];
+    r[3] = a.vy[j];
+  }
+  __device__ static void position(const Args &a, uint32_t j, double *p) {
+    p[0] = a.px[j];
+    p[1] = a.py[j];
+  }
+  struct Acc : BoidAccC {
+    __device__ void visit(const Args &a, const float *own, const float *nb) {
+      BoidAccC::visit(a, nb[0] - own[0], nb[1] - own[1], nb[2], nb[3]);
+    }
+  };
+  __device__ static void begin(const Args &, Acc &, const float *) {}
+  __device__ static void finish(const Args &a, Acc &s, const float *own, uint32_t gi,
+                                const double *p) {
+    boid_finish_c(a, s, p[0], p[1], own[2], own[3], gi);
+  }
+};
+
+template <class M, int TX, int TY>
+void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
   if (L.grid.reach <= 1) launch_tiled<M, TX, TY, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream);
   else if (L.grid.reach == 2) launch_tiled<M, TX, TY, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream);
   else launch_tiled<M, TX, TY, 1, 4>(a, L.grid, L.occupancy, L.block, L.stream);
+}
+
+void launch_c(const StepLaunch &L) {
+  BoidArgsC a{};
+  a.px = (const double *) L.in.c[0];
+  a.py = (const double *) L.in.c[1];
+  a.vx = (const float *) L.in.c[2];
+  a.vy = (const float *) L.in.c[3];
+  a.opx = (double *) L.out.c[0];
+  a.opy = (double *) L.out.c[1];
+  a.ovx = (float *) L.out.c[2];
+  a.ovy = (float *) L.out.c[3];
+  a.id = L.nb_id;
+  a.g = L.grid;
+  a.near_t = L.near_t;
+  a.sep_u = sqrtf_lt_threshold(L.params[P_SEPARATION_RADIUS]);
+  a.rc = L.rc;
+  a.n = L.n;
+  a.global_scale = (float) L.params[P_GLOBAL_SCALE];
+  a.steer = (float) L.params[P_STEER_SCALE];
+  a.collision = (float) L.params[P_COLLISION_SCALE];
+  a.match = (float) L.params[P_MATCH_SCALE];
+  a.max_speed = (float) L.params[P_MAX_SPEED];
+  a.time_scale = L.params[P_TIME_SCALE];
+  a.min_pos = L.params[P_MIN_POS];
+  a.max_pos = L.params[P_MAX_POS];
+  if (L.order == 1) {
+    hipLaunchKernelGGL((boids_step_c<true>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
+  } else if (L.tile == 1 && L.grid.reach <= 4) {
+    launch_tile_r<BoidPolicyC, 16, 8>(L, a);
+  } else if (L.tile == 2 && L.grid.reach <= 4) {
+    launch_tile_r<BoidPolicyC, 8, 8>(L, a);
+  } else if (L.tile == 3 && L.grid.reach <= 4) {
+    launch_tile_r<BoidPolicyC, 32, 8>(L, a);
+  } else {
+    hipLaunchKernelGGL((boids_step_c<false>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
+  }
 }
 
 template <typename F>
@@ -190,11 +348,11 @@ void launch_typed(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((boids_step<F, true>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   } else if (L.tile == 1 && L.grid.reach <= 4) {
-    launch_tile_r<F, 16, 8>(L, a);
+    launch_tile_r<BoidPolicy<F>, 16, 8>(L, a);
   } else if (L.tile == 2 && L.grid.reach <= 4) {
-    launch_tile_r<F, 8, 8>(L, a);
+    launch_tile_r<BoidPolicy<F>, 8, 8>(L, a);
   } else if (L.tile == 3 && L.grid.reach <= 4) {
-    launch_tile_r<F, 32, 8>(L, a);
+    launch_tile_r<BoidPolicy<F>, 32, 8>(L, a);
   } else {
     hipLaunchKernelGGL((boids_step<F, false>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   }
@@ -226,8 +384,8 @@ void records_typed(const double *p, std::vector<char> &out, size_t *count) {
 }
 
 int host_records(const double *p, int precision, std::vector<char> &out, size_t *count) {
-  if (precision == 0) records_typed<double>(p, out, count);
-  else records_typed<float>(p, out, count);
+  if (precision == ABL_PREC_F32) records_typed<float>(p, out, count);
+  else records_typed<double>(p, out, count);
   return ABL_OK;
 }
 
@@ -348,7 +506,8 @@ const EnsembleHooks ensemble_hooks = {ens_args_size, ens_tile_shape, ens_fill_ar
                                       ens_launch_multi, host_records};
 
 void launch_update_boid(const StepLaunch &L) {
-  if (L.precision == 0) launch_typed<double>(L);
+  if (L.precision == ABL_PREC_F32C) launch_c(L);
+  else if (L.precision == ABL_PREC_F64) launch_typed<double>(L);
   else launch_typed<float>(L);
 }
 
@@ -374,6 +533,7 @@ const ProgramDesc *program_boids2d() {
       host_init,
       nullptr,
       &ensemble_hooks,
+      true,
   };
   return &d;
 }

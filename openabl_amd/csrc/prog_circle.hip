@@ -135,12 +135,144 @@ struct CirclePolicy {
   }
 };
 
-template <typename F, int DIM, int TX, int TY, int TZ>
-void launch_tile_r(const StepLaunch &L, const CircleArgs<F, DIM> &a) {
-  using M = CirclePolicy<F, DIM>;
+// ---- FP32-compensated precision (ABL_PREC_F32C) -----------------------------
+// Positions stay FP64 in HBM and the step's sum of forces is added to them in
+// FP64.  The near loop runs in float on offsets nx.pos - in.pos (LDS records
+// are positions relative to the tile centre): float square sum against the
+// same threshold, fast float sqrt and reciprocal, and a Kahan-compensated
+// float sum of the forces (up to a few hundred neighbours in dense clusters).
+template <int DIM>
+struct CircleArgsC {
+  const double *p[3];
+  double *o[3];
+  const int32_t *id;
+  GridView g;
+  float near_t, rc;
+  int cap, n;
+  float r, k_rep, att_r2;  // att_r2 = 2.0 * r (folded in double, rounded)
+  float k_att;
+  double hi;                // W - 1.0
+};
+
+template <int DIM>
+struct CircleAccC {
+  KahanF f[DIM];
+  // o = nx.pos - in.pos
+  __device__ __forceinline__ void visit(const CircleArgsC<DIM> &a, const float *o) {
+    float s2 = o[0] * o[0] + o[1] * o[1];
+    if (DIM == 3) s2 = s2 + o[2] * o[2];
+    if (s2 > a.near_t) return;
+    if (s2 == 0.f) return;  // pos_dist == 0
+    const float pos_dist = __builtin_amdgcn_sqrtf(s2);
+    if (pos_dist - a.r > 0.f) {
+      const float s = a.k_att * (a.att_r2 - pos_dist) * rcp_f32(pos_dist);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) f[d].add(o[d] * s);
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) f[d].add(-o[d] * a.k_rep);
+    }
+  }
+};
+
+template <int DIM>
+__device__ __forceinline__ void circle_finish_c(const CircleArgsC<DIM> &a, const CircleAccC<DIM> &s,
+                                                const double *x, uint32_t i) {
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const double v = x[d] + s.f[d].value();
+    a.o[d][i] = v < 1.0 ? 1.0 : (v > a.hi ? a.hi : v);
+  }
+}
+
+template <int DIM, bool BY_ID>
+__global__ __launch_bounds__(1024) void circle_step_c(CircleArgsC<DIM> a) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = (int) (blk * blockDim.x + threadIdx.x);
+  if (i >= a.n) return;
+  double x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) x[d] = a.p[d][i];
+  const int3 c = cell_of<double>(a.g, x[0], x[1], x[2]);
+  CircleAccC<DIM> s;
+  auto body = [&](uint32_t j) {
+    float o[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) o[d] = (float) (a.p[d][j] - x[d]);
+    s.visit(a, o);
+  };
+  if (BY_ID) {
+    for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
+  } else {
+    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+  }
+  circle_finish_c<DIM>(a, s, x, (uint32_t) i);
+}
+
+template <int DIM_>
+struct CirclePolicyC {
+  using F = float;
+  static constexpr int DIM = DIM_, NS = DIM_;
+  static constexpr bool kRelative = true;
+  using Args = CircleArgsC<DIM>;
+  __device__ static void load(const Args &a, uint32_t j, float *r, const double *org) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) r[d] = (float) (a.p[d][j] - org[d]);
+  }
+  __device__ static void position(const Args &a, uint32_t j, double *p) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = a.p[d][j];
+  }
+  struct Acc : CircleAccC<DIM> {
+    __device__ void visit(const Args &a, const float *own, const float *nb) {
+      float o[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) o[d] = nb[d] - own[d];
+      CircleAccC<DIM>::visit(a, o);
+    }
+  };
+  __device__ static void begin(const Args &, Acc &, const float *) {}
+  __device__ static void finish(const Args &a, Acc &s, const float *, uint32_t i, const double *p) {
+    circle_finish_c<DIM>(a, s, p, i);
+  }
+};
+
+template <class M, int TX, int TY, int TZ>
+void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
   if (L.grid.reach <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream);
   else if (L.grid.reach == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream);
   else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream);
+}
+
+template <int DIM>
+void launch_c(const StepLaunch &L) {
+  CircleArgsC<DIM> a{};
+  for (int d = 0; d < DIM; d++) {
+    a.p[d] = (const double *) L.in.c[d];
+    a.o[d] = (double *) L.out.c[d];
+  }
+  a.id = L.nb_id;
+  a.g = L.grid;
+  a.near_t = L.near_t;
+  a.rc = L.rc;
+  a.n = L.n;
+  a.r = (float) L.params[P_R];
+  a.k_rep = (float) L.params[P_K_REP];
+  a.k_att = (float) L.params[P_K_ATT];
+  a.att_r2 = (float) (2.0 * L.params[P_R]);
+  a.hi = L.params[P_W] - 1.0;
+  using M = CirclePolicyC<DIM>;
+  const unsigned nb = grid_blocks(L.n, L.block);
+  if (L.order == 1) {
+    hipLaunchKernelGGL((circle_step_c<DIM, true>), nb, L.block, 0, L.stream, a);
+  } else if (L.tile != 0 && L.grid.reach <= 4) {
+    if (DIM == 3) launch_tile_r<M, 8, 4, 4>(L, a);
+    else if (L.tile == 2) launch_tile_r<M, 8, 8, 1>(L, a);
+    else if (L.tile == 3) launch_tile_r<M, 32, 8, 1>(L, a);
+    else launch_tile_r<M, 16, 8, 1>(L, a);
+  } else {
+    hipLaunchKernelGGL((circle_step_c<DIM, false>), nb, L.block, 0, L.stream, a);
+  }
 }
 
 template <typename F, int DIM>
@@ -163,10 +295,10 @@ void launch_typed(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
   } else if (L.tile != 0 && L.grid.reach <= 4) {
-    if (DIM == 3) launch_tile_r<F, DIM, 8, 4, 4>(L, a);
-    else if (L.tile == 2) launch_tile_r<F, DIM, 8, 8, 1>(L, a);
-    else if (L.tile == 3) launch_tile_r<F, DIM, 32, 8, 1>(L, a);
-    else launch_tile_r<F, DIM, 16, 8, 1>(L, a);
+    if (DIM == 3) launch_tile_r<CirclePolicy<F, DIM>, 8, 4, 4>(L, a);
+    else if (L.tile == 2) launch_tile_r<CirclePolicy<F, DIM>, 8, 8, 1>(L, a);
+    else if (L.tile == 3) launch_tile_r<CirclePolicy<F, DIM>, 32, 8, 1>(L, a);
+    else launch_tile_r<CirclePolicy<F, DIM>, 16, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((circle_step<F, DIM, false>), nb, L.block, 0, L.stream, a);
   }
@@ -254,15 +386,16 @@ int init3(abl_sim *sim, const double *p) {
 }
 
 int host_init2(abl_sim *sim, const double *p, int precision) {
-  return precision == 0 ? init2<double>(sim, p) : init2<float>(sim, p);
+  return precision == ABL_PREC_F32 ? init2<float>(sim, p) : init2<double>(sim, p);
 }
 int host_init3(abl_sim *sim, const double *p, int precision) {
-  return precision == 0 ? init3<double>(sim, p) : init3<float>(sim, p);
+  return precision == ABL_PREC_F32 ? init3<float>(sim, p) : init3<double>(sim, p);
 }
 
 template <int DIM>
 void launch_move_point(const StepLaunch &L) {
-  if (L.precision == 0) launch_typed<double, DIM>(L);
+  if (L.precision == ABL_PREC_F32C) launch_c<DIM>(L);
+  else if (L.precision == ABL_PREC_F64) launch_typed<double, DIM>(L);
   else launch_typed<float, DIM>(L);
 }
 
@@ -290,6 +423,9 @@ const ProgramDesc *program_circle() {
       {{"move_point", 0, 0, "r", 2.0, 0x3u, launch_move_point<2>}},
       env2,
       host_init2,
+      nullptr,
+      nullptr,
+      true,
   };
   return &d;
 }
@@ -304,6 +440,9 @@ const ProgramDesc *program_circle3d() {
       {{"move_point", 0, 0, "r", 2.0, 0x7u, launch_move_point<3>}},
       env3,
       host_init3,
+      nullptr,
+      nullptr,
+      true,
   };
   return &d;
 }

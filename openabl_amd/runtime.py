@@ -292,7 +292,7 @@ def record_dtype(members, precision: str = "f64") -> np.dtype:
     """numpy dtype of a generated agent struct: members [(name, kind)] with
     kind 'bool', 'int', 'float', 'float2', 'float3', laid out by the C rules
     the C backend's struct follows (src/backend/CPrinter.cpp:250-253)."""
-    fb = 8 if precision == "f64" else 4
+    fb = 4 if precision == "f32" else 8  # f32c keeps the FP64 struct layout
     fcode = "<f8" if fb == 8 else "<f4"
     names, formats, offsets = [], [], []
     off, amax = 0, 1
