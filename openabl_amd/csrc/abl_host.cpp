@@ -16,7 +16,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cfloat>
 #include <climits>
+#include <cmath>
+#include <cstring>
 #include <cstdint>
 #include <mutex>
 
@@ -166,6 +169,25 @@ float sqrtf_lt_threshold(double r) {
     if ((double) sqrtf(bits_to_float(mid)) < r) lo = mid; else hi = mid;
   }
   return bits_to_float(hi);
+}
+
+// Largest double d with (float) d <= t under round-to-nearest-even, so that
+// for every double s:  (float) s <= t  <=>  s <= double_le_threshold(t).
+// Between t and the next float up, (float) s is t below the midpoint m and
+// at m itself when t's significand is even; m is exact in double.
+double double_le_threshold(float t) {
+  if (std::isnan(t)) return (double) t;
+  if (std::isinf(t)) return t > 0 ? HUGE_VAL : -(double) FLT_MAX - std::ldexp(1.0, 103);
+  double m;
+  if (t == FLT_MAX) {
+    m = (double) FLT_MAX + std::ldexp(1.0, 103);  // half an ulp above the top float
+  } else {
+    const float up = std::nextafter(t, HUGE_VALF);
+    m = ((double) t + (double) up) * 0.5;
+  }
+  uint32_t bits;
+  std::memcpy(&bits, &t, 4);
+  return (bits & 1u) ? std::nextafter(m, -HUGE_VAL) : m;
 }
 
 }  // namespace abl

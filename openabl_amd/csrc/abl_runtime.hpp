@@ -154,6 +154,7 @@ __host__ __device__ inline F counter_random(F lo, F hi, uint64_t bits) {
 // Host-side helpers (abl_host.cpp).
 float sqrtf_le_threshold(double r);  // max float f >= 0 with sqrtf(f) <= r
 float sqrtf_lt_threshold(double r);  // min float f >= 0 with sqrtf(f) >= r
+double double_le_threshold(float t);  // max double d with (float) d <= t
 
 // Program registry (one entry per prog_*.hip).
 const ProgramDesc *program_circle();

@@ -11,9 +11,13 @@
 //   steer/match/avoid terms, new_velocity = velocity + change * global_scale,
 //   speed clamp v / |v| * max_speed, new_pos = pos + v * time_scale,
 //   boundPosition wrap.
-// Both distance comparisons are done on the float-rounded square sum against
-// host-computed thresholds (exact: sqrtf is monotone), so the loop takes no
-// square root; only |new_velocity| needs one.
+// Both distance comparisons are done on the square sum against host-computed
+// thresholds, so the loop takes no square root (only |new_velocity| needs
+// one).  sqrtf is monotone, so dist <= r <=> (float) s <= T and dist < r_sep
+// <=> (float) s < U for float thresholds T, U; in FP64 the rounding to float
+// is folded into the threshold too: (float) s <= T <=> s <= Td for the
+// largest double Td that rounds to at most T.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -33,8 +37,8 @@ struct BoidArgs {
   F *opx, *opy, *ovx, *ovy;    // out
   const int32_t *id;
   GridView g;
-  float near_t;   // dist <= interaction_radius  <=>  f <= near_t
-  float sep_u;    // dist <  separation_radius   <=>  f <  sep_u
+  F near_s;       // dist <= interaction_radius  <=>  s <= near_s
+  F sep_s;        // dist <  separation_radius   <=>  s <= sep_s
   float rc;       // radius in cell units (row narrowing)
   int cap;        // LDS records per tile
   int n;
@@ -47,14 +51,17 @@ struct BoidAcc {
   F gvx = 0, gvy = 0, gcx = 0, gcy = 0, ccx = 0, ccy = 0;
   int icount = 0, ccount = 0;
   __device__ __forceinline__ void visit(const BoidArgs<F> &a, F x, F y, F qx, F qy, F qvx, F qvy) {
-    const float f = dist_sq_f32<F>(qx, qy, x, y);
-    if (f > a.near_t) return;
+    // s = (nx.pos - in.pos) . (nx.pos - in.pos), as dist() forms it
+    const F dx = qx - x;
+    const F dy = qy - y;
+    const F s = dx * dx + dy * dy;
+    if (s > a.near_s) return;
     gcx = gcx + qx;
     gcy = gcy + qy;
     gvx = gvx + qvx;
     gvy = gvy + qvy;
     icount += 1;
-    if (f < a.sep_u) {
+    if (s <= a.sep_s) {
       ccx = ccx + qx;
       ccy = ccy + qy;
       ccount += 1;
@@ -153,6 +160,21 @@ struct BoidPolicy {
     boid_finish<F>(a, s, own[0], own[1], own[2], own[3], gi);
   }
 };
+
+// Square-sum thresholds: near_t = max{f : sqrtf(f) <= interaction_radius};
+// the separation test sqrtf(f) < r_sep is f < U = f <= prev(U).
+template <typename F>
+void set_thresholds(BoidArgs<F> &a, float near_t, double sep_radius) {
+  const float u = sqrtf_lt_threshold((double) (F) sep_radius);
+  const float u_le = std::nextafter(u, -HUGE_VALF);
+  if (sizeof(F) == 8) {
+    a.near_s = (F) double_le_threshold(near_t);
+    a.sep_s = (F) double_le_threshold(u_le);
+  } else {
+    a.near_s = (F) near_t;
+    a.sep_s = (F) u_le;
+  }
+}
 
 // ---- FP32-compensated precision (ABL_PREC_F32C) -----------------------------
 // Positions stay FP64 in HBM and are integrated in FP64; velocity is stored
@@ -333,8 +355,7 @@ void launch_typed(const StepLaunch &L) {
   a.ovy = (F *) L.out.c[3];
   a.id = L.nb_id;
   a.g = L.grid;
-  a.near_t = L.near_t;
-  a.sep_u = sqrtf_lt_threshold((double) (F) L.params[P_SEPARATION_RADIUS]);
+  set_thresholds(a, L.near_t, L.params[P_SEPARATION_RADIUS]);
   a.rc = L.rc;
   a.n = L.n;
   a.time_scale = (F) L.params[P_TIME_SCALE];
@@ -411,8 +432,7 @@ BoidArgs<F> make_args(const double *p, const ColSet &in, const ColSet &out, cons
   a.ovx = (F *) out.c[2];
   a.ovy = (F *) out.c[3];
   a.g = g;
-  a.near_t = near_t;
-  a.sep_u = sqrtf_lt_threshold((double) (F) p[P_SEPARATION_RADIUS]);
+  set_thresholds(a, near_t, p[P_SEPARATION_RADIUS]);
   a.rc = rc;
   a.n = n;
   a.time_scale = (F) p[P_TIME_SCALE];

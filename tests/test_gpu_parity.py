@@ -233,3 +233,61 @@ def test_errors_are_reported():
         sim.set_param("nonexistent", 1.0)
     with pytest.raises(AblError):
         sim.set_config("hip.block", "100")
+
+
+def _float_threshold_le(r: float) -> np.float32:
+    """max float32 f >= 0 with float(sqrt(f)) <= r (numpy's float32 sqrt is
+    correctly rounded)."""
+    lo, hi = 0, 0x7F800000
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        f = np.array([mid], dtype=np.uint32).view(np.float32)[0]
+        if float(np.sqrt(f)) <= r:
+            lo = mid
+        else:
+            hi = mid
+    return np.array([lo], dtype=np.uint32).view(np.float32)[0]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("order", ["cell", "index"])
+def test_boids_radius_boundary(prec, order):
+    """Isolated pairs of boids whose squared distance straddles the float
+    rounding boundary of the interaction radius (10) and of the separation
+    radius (2): the update_boid kernel accepts exactly the pairs the C
+    backend's (double) sqrtf((float) s) <= r (resp. < r) accepts, so one step
+    is bit-exact (two-term sums do not depend on order)."""
+    params = boids2d_params(num_agents=65536)  # 1000 x 1000 box
+    dt = dtype_of("boids2d", prec)
+    ft = np.float64 if prec == "f64" else np.float32
+    offsets = []
+    for r in (10.0, 2.0):
+        t = _float_threshold_le(r)
+        t_up = np.nextafter(t, np.float32(np.inf))
+        for target in (float(t), (float(t) + float(t_up)) / 2, float(t_up)):
+            d = ft(np.sqrt(target))
+            for j in range(-3, 4):
+                dd = d
+                for _ in range(abs(j)):
+                    dd = np.nextafter(dd, ft(np.inf) if j > 0 else ft(-np.inf))
+                offsets.append(dd)
+    pts, vel = [], []
+    rng = np.random.default_rng(11)
+    # A on the x = 0 edge so that B's x coordinate is the offset itself
+    # (no rounding of x0 + dx), pairs 20 apart in y
+    for k, dx in enumerate(offsets):
+        y0 = ft(10 + 20 * k)
+        pts += [(ft(0), y0), (dx, y0)]
+        vel += [tuple(rng.uniform(-1, 1, 2)), tuple(rng.uniform(-1, 1, 2))]
+    assert 10 + 20 * len(offsets) <= params["max_pos"]
+    recs = np.zeros(len(pts), dtype=dt)
+    recs["pos"] = np.array(pts, dtype=ft)
+    recs["velocity"] = np.array(vel, dtype=ft)
+    o = Oracle("boids2d", prec).set(**params)
+    o.set_agents(recs)
+    o.simulate(1)
+    from openabl_amd import Simulation
+    sim = Simulation("boids2d", prec, params=params, config={"hip.order": order})
+    sim.upload(recs)
+    sim.simulate(1)
+    assert field_bytes(state(sim, "boids2d", prec)) == field_bytes(o.agents())
