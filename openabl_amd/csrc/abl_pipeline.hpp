@@ -1,10 +1,14 @@
 // abl_pipeline.hpp -- the grid pipeline's scan and sort kernels, shared by
 // the single-simulation runtime (abl_runtime.hip) and the ensemble
 // (abl_ensemble.hip).
-//   scan     exclusive prefix of per-cell counts (wave64 shuffles, 3 passes)
+//   scan     exclusive prefix of per-cell counts (wave64 shuffles): three
+//            passes (k_scan_tiles/totals/add) or two (k_tile_totals,
+//            k_scan_apply)
 //   scatter  slot = start[c] + --count[c]   (count returns to zero)
 //   rank     stable order inside each cell by id, fused with the gather of
 //            every column into the other buffer
+//   delta    re-sort of an already sorted order after a step (k_bin_delta,
+//            k_place): atomics only for the agents that changed cell
 #pragma once
 
 #include "abl_device.hpp"
@@ -81,10 +85,25 @@ struct GatherCols {
   int ncols;
 };
 
+__device__ __forceinline__ void gather_cols(const GatherCols &g, uint32_t src, uint32_t dst) {
+  for (int c = 0; c < g.ncols; c++) {
+    switch (g.elem[c]) {
+      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
+      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
+      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
+    }
+  }
+}
+
 // Stable placement inside each cell by id, fused with the column gather.
+// Optionally records the sorted keys (skey_out) and restores the per-cell
+// counts the scatter consumed (count_out, written by each cell's rank-0
+// agent; empty cells are already zero).
 static __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const int32_t *__restrict__ tmp_id,
                               const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
-                              GatherCols g, int32_t *__restrict__ id_out, int n) {
+                              GatherCols g, int32_t *__restrict__ id_out, int n,
+                              uint32_t *__restrict__ skey_out = nullptr,
+                              uint32_t *__restrict__ count_out = nullptr) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t src = tmp_idx[s];
@@ -95,13 +114,136 @@ static __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const
   for (uint32_t t = a; t < b; t++) rank += tmp_id[t] < my ? 1u : 0u;
   const uint32_t dst = a + rank;
   id_out[dst] = my;
-  for (int c = 0; c < g.ncols; c++) {
-    switch (g.elem[c]) {
-      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
-      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
-      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
+  if (skey_out) skey_out[dst] = k;
+  if (count_out && rank == 0) count_out[k] = b - a;
+  gather_cols(g, src, dst);
+}
+
+// ---- two-kernel scan of per-cell counts --------------------------------------
+// k_tile_totals: the total of every tile of kScanTile cells.  k_scan_apply:
+// each tile's exclusive scan plus the sum of the totals of the tiles before
+// it (each block adds those up itself), and start[ncells].
+
+static __global__ __launch_bounds__(kScanBlock) void k_tile_totals(const uint32_t *__restrict__ count,
+                                                                   uint32_t *__restrict__ bsum,
+                                                                   int ncells) {
+  __shared__ uint32_t lds[kWave + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) sum += base + k < ncells ? count[base + k] : 0u;
+  uint32_t total;
+  (void) block_excl_scan(sum, lds, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ count,
+                                                                  const uint32_t *__restrict__ bsum,
+                                                                  uint32_t *__restrict__ start,
+                                                                  int ncells) {
+  __shared__ uint32_t lds[kWave + 1];
+  uint32_t part = 0;
+  for (int t = threadIdx.x; t < (int) blockIdx.x; t += blockDim.x) part += bsum[t];
+  uint32_t carry;
+  (void) block_excl_scan(part, lds, &carry);
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = base + k < ncells ? count[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan(sum, lds, &total) + carry;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < ncells) start[base + k] = pre;
+    pre += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = carry + total;
+}
+
+// ---- delta re-sort -------------------------------------------------------------
+// Between two builds on the same grid the stored order is sorted by (cell,
+// id) and few agents change cell.  k_bin_delta computes every slot's new
+// cell; an agent whose cell changed (a mover) moves one count (and tile
+// total) from its old cell to its new one, tags both cells dirty for this
+// build generation and pushes itself on its new cell's mover list.  k_place
+// then puts every agent at its place in the (cell, id) order: in a clean cell
+// the agents and their order are unchanged, so an agent keeps its offset from
+// the cell's start; in a dirty cell its place is its rank by id among the
+// cell's stayers and incoming movers.  The result is the full sort's.
+
+struct MoverLists {
+  uint32_t *dirty;                 // per cell: generation of the last build that touched it
+  unsigned long long *head;        // per cell: (generation << 32) | first mover, newest first
+  unsigned long long *next;        // per mover: the head it replaced
+  uint32_t *slot;                  // per mover: its slot in the stored order
+  uint32_t *nmov;                  // two counters, used by alternate generations
+};
+
+template <typename P>
+static __global__ void k_bin_delta(const P *__restrict__ px, const P *__restrict__ py,
+                                   const P *__restrict__ pz, GridView g,
+                                   const uint32_t *__restrict__ skey, uint32_t *__restrict__ key,
+                                   uint32_t *__restrict__ count, uint32_t *__restrict__ bsum,
+                                   MoverLists ml, uint32_t gen, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int3 c = cell_of<P>(g, px[i], py[i], g.dim == 3 ? pz[i] : (P) 0);
+  const uint32_t k = ((uint32_t) c.z * (uint32_t) g.n[1] + (uint32_t) c.y) * (uint32_t) g.n[0] +
+                     (uint32_t) c.x;
+  key[i] = k;
+  const uint32_t o = skey[i];
+  if (k != o) {
+    atomicAdd(&count[k], 1u);
+    atomicSub(&count[o], 1u);
+    atomicAdd(&bsum[k / kScanTile], 1u);
+    atomicSub(&bsum[o / kScanTile], 1u);
+    ml.dirty[k] = gen;
+    ml.dirty[o] = gen;
+    const uint32_t m = atomicAdd(&ml.nmov[gen & 1u], 1u);
+    if (m < (uint32_t) n) {  // always true when the counter started at zero
+      ml.slot[m] = (uint32_t) i;
+      ml.next[m] = atomicExch(&ml.head[k], ((unsigned long long) gen << 32) | m);
     }
   }
+}
+
+static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t *__restrict__ skey_in,
+                               uint32_t *__restrict__ skey_out,
+                               const uint32_t *__restrict__ start_old,
+                               const uint32_t *__restrict__ start_new, MoverLists ml, uint32_t gen,
+                               GatherCols g, const int32_t *__restrict__ id_in,
+                               int32_t *__restrict__ id_out, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  // the counter the next generation uses starts from zero
+  if (s == 0) ml.nmov[(gen + 1u) & 1u] = 0u;
+  if (s >= n) return;
+  const uint32_t k = key[s];
+  uint32_t dst;
+  if (k == skey_in[s] && ml.dirty[k] != gen) {
+    dst = start_new[k] + ((uint32_t) s - start_old[k]);
+  } else {
+    const int32_t my = id_in[s];
+    uint32_t rank = 0;
+    // stayers: the cell's old range, less the agents that left it
+    const uint32_t a = start_old[k], b = start_old[k + 1];
+    for (uint32_t t = a; t < b; t++) rank += (key[t] == k && id_in[t] < my) ? 1u : 0u;
+    // incoming movers of this generation (each pushed once: the list ends)
+    unsigned long long h = ml.head[k];
+    while ((uint32_t) (h >> 32) == gen) {
+      const uint32_t m = (uint32_t) h;
+      rank += id_in[ml.slot[m]] < my ? 1u : 0u;
+      h = ml.next[m];
+    }
+    dst = start_new[k] + rank;
+  }
+  if (dst >= (uint32_t) n) return;  // unreachable while the counts are consistent
+  skey_out[dst] = k;
+  id_out[dst] = id_in[s];
+  gather_cols(g, (uint32_t) s, dst);
 }
 
 

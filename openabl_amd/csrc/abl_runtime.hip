@@ -107,14 +107,29 @@ struct AgentStore {
   int cur[kMaxCols] = {};
   int32_t *id[2] = {nullptr, nullptr};
   int id_cur = 0;
-  // grid
-  uint32_t *key = nullptr, *count = nullptr, *start = nullptr, *bsum = nullptr;
+  // grid: per-agent keys and sort scratch, per-cell counts and (two) start
+  // arrays, per-tile totals of the counts
+  uint32_t *key = nullptr, *count = nullptr, *bsum = nullptr;
+  uint32_t *startb[2] = {nullptr, nullptr};
+  int start_cur = 0;
   uint32_t *tmp_idx = nullptr;
   int32_t *tmp_id = nullptr;
   size_t ncells = 0, cells_cap = 0;
   bool grid_valid = false;
   GridView gv{};
   double cell = 0.0;
+  // delta re-sort (abl_pipeline.hpp): sorted keys of the stored order, mover
+  // lists, and whether the stored order is the (cell, id) order of the last
+  // build (same grid, same agents) with count/bsum/skey describing it
+  uint32_t *skey[2] = {nullptr, nullptr};
+  int skey_cur = 0;
+  uint32_t *dirty = nullptr, *mslot = nullptr, *nmov = nullptr;
+  unsigned long long *mhead = nullptr, *mnext = nullptr;
+  uint32_t gen = 0;
+  bool sorted = false;
+  bool nmov_clean = false;
+  GridView sgv{};
+  size_t sorted_n = 0;
   // run-time add/remove: ids stop being 0..n-1 once agents die or are born
   int64_t next_id = 0;
   bool ids_dense = true;
@@ -135,6 +150,8 @@ struct abl_sim {
   int cell_scale = 1;
   int order = 0;
   int tile = 1;
+  bool grid_delta = true;  // "hip.grid": delta re-sort when the stored order allows it
+  float lds_slack = 2.0f;  // "hip.lds_slack": LDS tile records per expected halo agent
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -200,12 +217,16 @@ static void free_store(AgentStore &s) {
     if (s.id[b]) (void) hipFree(s.id[b]);
     s.id[b] = nullptr;
   }
-  void *grid_ptrs[] = {s.key, s.count, s.start, s.bsum, s.tmp_idx, s.tmp_id};
+  void *grid_ptrs[] = {s.key, s.count, s.startb[0], s.startb[1], s.bsum, s.tmp_idx, s.tmp_id,
+                       s.skey[0], s.skey[1], s.dirty, s.mslot, s.nmov, s.mhead, s.mnext};
   for (void *p : grid_ptrs) {
     if (p) (void) hipFree(p);
   }
-  s.key = s.count = s.start = s.bsum = s.tmp_idx = nullptr;
+  s.key = s.count = s.startb[0] = s.startb[1] = s.bsum = s.tmp_idx = nullptr;
   s.tmp_id = nullptr;
+  s.skey[0] = s.skey[1] = s.dirty = s.mslot = s.nmov = nullptr;
+  s.mhead = s.mnext = nullptr;
+  s.sorted = false;
   for (int c = 0; c < kMaxCols; c++) {
     if (s.bmem[c]) (void) hipFree(s.bmem[c]);
     s.bmem[c] = nullptr;
@@ -220,8 +241,8 @@ static size_t store_bytes(const AgentStore &s) {
   size_t b = 0;
   for (const Column &c : s.cols) b += 2 * s.cap * (size_t) c.elem;
   b += 2 * s.cap * 4;                      // ids
-  b += s.cap * (4 + 4 + 4);                // key, tmp_idx, tmp_id
-  b += s.cells_cap * 4 * 2 + 4096 * 4;     // count, start, bsum
+  b += s.cap * (4 + 4 + 4 + 8 + 4 + 8);    // key, tmp_idx, tmp_id, skey x2, mover slot, next
+  b += s.cells_cap * (4 + 8 + 4 + 8) + 4096 * 4;  // count, start x2, dirty, head; bsum
   return b;
 }
 
@@ -253,15 +274,23 @@ static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
     }
     s.id[b] = p;
   }
-  uint32_t *ptrs[2] = {s.key, s.tmp_idx};
-  for (uint32_t *p : ptrs) {
+  void *ptrs[] = {s.key, s.tmp_idx, s.tmp_id, s.skey[0], s.skey[1], s.mslot, s.mnext};
+  for (void *p : ptrs) {
     if (p) HIP_TRY(hipFree(p));
   }
-  if (s.tmp_id) HIP_TRY(hipFree(s.tmp_id));
   HIP_TRY(hipMalloc(&s.key, cap * 4));
   HIP_TRY(hipMalloc(&s.tmp_idx, cap * 4));
   HIP_TRY(hipMalloc(&s.tmp_id, cap * 4));
+  HIP_TRY(hipMalloc(&s.skey[0], cap * 4));
+  HIP_TRY(hipMalloc(&s.skey[1], cap * 4));
+  HIP_TRY(hipMalloc(&s.mslot, cap * 4));
+  HIP_TRY(hipMalloc(&s.mnext, cap * 8));
+  if (!s.nmov) {
+    HIP_TRY(hipMalloc(&s.nmov, 2 * 4));
+    HIP_TRY(hipMemsetAsync(s.nmov, 0, 2 * 4, sim->stream));
+  }
   s.cap = cap;
+  s.sorted = false;
   return ABL_OK;
 }
 
@@ -484,66 +513,127 @@ static int setup_grid(abl_sim *sim, AgentStore &s) {
   g.inv_cell = 1.0 / cell;
   s.cell = cell;
   if (ncells + 1 > s.cells_cap) {
-    if (s.count) HIP_TRY(hipFree(s.count));
-    if (s.start) HIP_TRY(hipFree(s.start));
-    if (s.bsum) HIP_TRY(hipFree(s.bsum));
-    s.count = s.start = s.bsum = nullptr;
+    void *ptrs[] = {s.count, s.startb[0], s.startb[1], s.bsum, s.dirty, s.mhead};
+    for (void *p : ptrs) {
+      if (p) HIP_TRY(hipFree(p));
+    }
+    s.count = s.startb[0] = s.startb[1] = s.bsum = s.dirty = nullptr;
+    s.mhead = nullptr;
     HIP_TRY(hipMalloc(&s.count, (ncells + 1) * 4));
-    HIP_TRY(hipMalloc(&s.start, (ncells + 1) * 4));
+    HIP_TRY(hipMalloc(&s.startb[0], (ncells + 1) * 4));
+    HIP_TRY(hipMalloc(&s.startb[1], (ncells + 1) * 4));
     size_t ntiles = (ncells + kScanTile - 1) / kScanTile;
     HIP_TRY(hipMalloc(&s.bsum, std::max<size_t>(ntiles, 1) * 4));
+    HIP_TRY(hipMalloc(&s.dirty, (ncells + 1) * 4));
+    HIP_TRY(hipMalloc(&s.mhead, (ncells + 1) * 8));
     HIP_TRY(hipMemsetAsync(s.count, 0, (ncells + 1) * 4, sim->stream));
+    HIP_TRY(hipMemsetAsync(s.dirty, 0, (ncells + 1) * 4, sim->stream));
+    HIP_TRY(hipMemsetAsync(s.mhead, 0, (ncells + 1) * 8, sim->stream));
     s.cells_cap = ncells + 1;
+    s.sorted = false;
   }
   s.ncells = ncells;
-  g.start = s.start;
+  g.start = s.startb[s.start_cur];
   s.gv = g;
   return ABL_OK;
 }
 
+static bool same_grid(const GridView &a, const GridView &b) {
+  for (int d = 0; d < 3; d++) {
+    if (a.lo[d] != b.lo[d] || a.n[d] != b.n[d]) return false;
+  }
+  return a.inv_cell == b.inv_cell && a.dim == b.dim;
+}
+
+template <typename P>
+static void launch_bin(abl_sim *sim, AgentStore &s, bool delta, int n) {
+  const int pc = s.pos_col;
+  const P *px = (const P *) col_cur(s, pc);
+  const P *py = (const P *) col_cur(s, pc + 1);
+  const P *pz = (const P *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr);
+  if (delta) {
+    MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
+    hipLaunchKernelGGL(k_bin_delta<P>, grid_blocks(n, 256), 256, 0, sim->stream, px, py, pz, s.gv,
+                       s.skey[s.skey_cur], s.key, s.count, s.bsum, ml, s.gen, n);
+  } else {
+    hipLaunchKernelGGL(k_bin<P>, grid_blocks(n, 256), 256, 0, sim->stream, px, py, pz, s.gv, s.key,
+                       s.count, n);
+  }
+}
+
+// Cell-sort the agents of `s` (stable by id inside a cell) and build the cell
+// offsets.  Full build: bin with per-agent atomics, scan, scatter, rank by id
+// and gather.  Delta build (the stored order is the (cell, id) order of the
+// last build on the same grid): only agents that changed cell touch atomics,
+// and one pass places every agent (abl_pipeline.hpp).  Both give the same
+// order, a function of positions and ids only.
 static int build_grid(abl_sim *sim, AgentStore &s) {
   if (s.pos_col < 0) return fail(ABL_ERR_ARG, "near loop over an agent type without position");
   int rc = setup_grid(sim, s);
   if (rc) return rc;
   const int n = (int) s.n;
-  const GridView &g = s.gv;
   const int nc = (int) s.ncells;
   hipStream_t st = sim->stream;
-  const int blk = 256;
-  if (n > 0) {
-    const int pc = s.pos_col;
-    if (s.cols[pc].elem == 8) {
-      hipLaunchKernelGGL(k_bin<double>, grid_blocks(n, blk), blk, 0, st,
-                         (const double *) col_cur(s, pc), (const double *) col_cur(s, pc + 1),
-                         (const double *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr), g, s.key,
-                         s.count, n);
-    } else {
-      hipLaunchKernelGGL(k_bin<float>, grid_blocks(n, blk), blk, 0, st,
-                         (const float *) col_cur(s, pc), (const float *) col_cur(s, pc + 1),
-                         (const float *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr), g, s.key,
-                         s.count, n);
-    }
-  }
+  const bool delta = sim->grid_delta && n > 0 && s.nmov && s.sorted && s.sorted_n == s.n &&
+                     same_grid(s.gv, s.sgv);
+  s.gen += 1;
+  if (s.gen == 0) s.gen = 1;  // tag 0 marks never-touched cells and lists
   const int ntiles = (nc + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL((k_scan_tiles<uint32_t, false>), ntiles, kScanBlock, 0, st, s.count, s.start, s.bsum, nc);
-  hipLaunchKernelGGL(k_scan_totals, 1, 1024, 0, st, s.bsum, ntiles, s.start, nc);
-  hipLaunchKernelGGL(k_scan_add, grid_blocks(nc, 256), 256, 0, st, s.start, s.bsum, nc);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, grid_blocks(n, blk), blk, 0, st, s.key, s.count, s.start,
-                       s.id[s.id_cur], s.tmp_idx, s.tmp_id, n);
-    GatherCols gc{};
-    gc.ncols = (int) s.cols.size();
-    for (int c = 0; c < gc.ncols; c++) {
-      gc.src[c] = col_cur(s, c);
-      gc.dst[c] = col_other(s, c);
-      gc.elem[c] = s.cols[c].elem;
+  uint32_t *start_old = s.startb[s.start_cur];
+  uint32_t *start_new = s.startb[1 - s.start_cur];
+  const bool dbl = s.cols[s.pos_col].elem == 8;
+  GatherCols gc{};
+  gc.ncols = (int) s.cols.size();
+  for (int c = 0; c < gc.ncols; c++) {
+    gc.src[c] = col_cur(s, c);
+    gc.dst[c] = col_other(s, c);
+    gc.elem[c] = s.cols[c].elem;
+  }
+  if (delta) {
+    if (!s.nmov_clean) HIP_TRY(hipMemsetAsync(s.nmov, 0, 2 * 4, st));
+    if (n > 0) {
+      if (dbl) launch_bin<double>(sim, s, true, n);
+      else launch_bin<float>(sim, s, true, n);
     }
-    hipLaunchKernelGGL(k_rank_gather, grid_blocks(n, blk), blk, 0, st, s.tmp_idx, s.tmp_id,
-                       s.key, s.start, gc, s.id[1 - s.id_cur], n);
-    for (int c = 0; c < gc.ncols; c++) s.cur[c] ^= 1;
-    s.id_cur ^= 1;
+    hipLaunchKernelGGL(k_scan_apply, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum,
+                       start_new, nc);
+    if (n > 0) {
+      MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
+      hipLaunchKernelGGL(k_place, grid_blocks(n, 256), 256, 0, st, s.key, s.skey[s.skey_cur],
+                         s.skey[1 - s.skey_cur], start_old, start_new, ml, s.gen, gc,
+                         s.id[s.id_cur], s.id[1 - s.id_cur], n);
+    }
+    s.nmov_clean = n > 0;
+  } else {
+    // the counts of the previous build are still in `count`
+    HIP_TRY(hipMemsetAsync(s.count, 0, (size_t) (nc + 1) * 4, st));
+    if (n > 0) {
+      if (dbl) launch_bin<double>(sim, s, false, n);
+      else launch_bin<float>(sim, s, false, n);
+    }
+    hipLaunchKernelGGL(k_tile_totals, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum, nc);
+    hipLaunchKernelGGL(k_scan_apply, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum,
+                       start_new, nc);
+    if (n > 0) {
+      hipLaunchKernelGGL(k_scatter, grid_blocks(n, 256), 256, 0, st, s.key, s.count, start_new,
+                         s.id[s.id_cur], s.tmp_idx, s.tmp_id, n);
+      hipLaunchKernelGGL(k_rank_gather, grid_blocks(n, 256), 256, 0, st, s.tmp_idx, s.tmp_id,
+                         s.key, start_new, gc, s.id[1 - s.id_cur], n, s.skey[1 - s.skey_cur],
+                         s.count);
+    }
+    s.nmov_clean = false;
   }
   HIP_TRY(hipGetLastError());
+  if (n > 0) {
+    for (int c = 0; c < gc.ncols; c++) s.cur[c] ^= 1;
+    s.id_cur ^= 1;
+    s.skey_cur ^= 1;
+  }
+  s.start_cur ^= 1;
+  s.gv.start = start_new;
+  s.sgv = s.gv;
+  s.sorted = true;
+  s.sorted_n = s.n;
   s.grid_valid = true;
   return ABL_OK;
 }
@@ -695,6 +785,14 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     int t = std::atoi(value);
     if (t < 0 || t > 3) return fail(ABL_ERR_ARG, "hip.tile must be 0..3");
     sim->tile = t;
+  } else if (k == "hip.lds_slack") {
+    const double f = std::atof(value);
+    if (!(f >= 1.0 && f <= 4.0)) return fail(ABL_ERR_ARG, "hip.lds_slack must be 1..4");
+    sim->lds_slack = (float) f;
+  } else if (k == "hip.grid") {
+    if (v == "delta") sim->grid_delta = true;
+    else if (v == "full") sim->grid_delta = false;
+    else return fail(ABL_ERR_ARG, "hip.grid must be delta or full");
   } else if (k == "hip.seed") {
     sim->seed = std::strtoull(value, nullptr, 0);
   } else if (k == "hip.log_csv") {
@@ -743,6 +841,7 @@ extern "C" int abl_hip_upload(abl_sim *sim, int t, const void *aos, size_t count
   if (rc) return rc;
   s.n = count;
   s.grid_valid = false;
+  s.sorted = false;
   s.ids_dense = true;
   s.next_id = (int64_t) count;
   if (count == 0) return ABL_OK;
@@ -939,6 +1038,7 @@ static int apply_removals(abl_sim *sim, AgentStore &a, int n) {
   a.n = alive;
   a.ids_dense = false;
   a.grid_valid = false;
+  a.sorted = false;
   return ABL_OK;
 }
 
@@ -965,6 +1065,7 @@ static int apply_births(abl_sim *sim, AgentStore &parent, AgentStore &child, int
   child.next_id += nb;
   child.ids_dense = false;
   child.grid_valid = false;
+  child.sorted = false;
   return ABL_OK;
 }
 
@@ -1014,6 +1115,7 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
     L.grid.reach = reach_for(b, r);
     L.near_t = sqrtf_le_threshold(r);
     L.tile = sim->tile;
+    L.lds_slack = sim->lds_slack;
     L.occupancy = b.ncells ? (float) ((double) b.n / (double) b.ncells) : 0.f;
     L.rc = (float) (r / b.cell * (1.0 + 1.0 / 1048576.0));
     if (sim->order == 1) {

@@ -118,6 +118,7 @@ struct StepLaunch {
   int order;           // 0 = cell order, 1 = reference index order
   int tile;            // LDS tile variant (0: untiled thread-per-agent kernel)
   float occupancy;     // near-type agents per cell (sizes the LDS tile budget)
+  float lds_slack;     // LDS records per expected halo agent
   float rc;            // near radius in cell units, widened by 2^-20
   // run-time add()/removeCurrent() and random() inside steps
   const int32_t *self_id;  // ids of the stepping agents

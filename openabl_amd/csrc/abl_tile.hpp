@@ -185,7 +185,11 @@ namespace abl {
 // centre (FP32-compensated precision).  It then supplies instead
 //   static void load(const Args &, uint32_t slot, F *rec, const double org[3])
 //   static void position(const Args &, uint32_t slot, double p[3])  -- FP64 position
+//   static void begin(const Args &, Acc &, const F *own, const double p[3], const double org[3])
 //   static void finish(const Args &, Acc &, const F *own, uint32_t slot, const double p[3])
+// begin() receives the agent's FP64 position and the origin, so the policy
+// can remove the rounding of the agent's own float record from every offset
+// (otherwise one systematic error shared by all of its neighbour terms).
 template <class M, class = void>
 struct policy_relative {
   static constexpr bool value = false;
@@ -279,7 +283,8 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
     const float uy = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) c.y);
     const float uz = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) c.z) : 0.f;
     typename M::Acc acc;
-    M::begin(a, acc, own);
+    if constexpr (REL) M::begin(a, acc, own, pd, org);
+    else M::begin(a, acc, own);
     const int zr = DIM == 3 ? R : 0;
     for (int dz = -zr; dz <= zr; dz++) {
       const int zz = c.z + dz;
@@ -296,7 +301,27 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
         const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
         const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
         if (!ovf) {
-          for (uint32_t l = l0; l < l1; l++) acc.visit(a, own, rec + (size_t) NS * l);
+          if (l0 < l1) {
+            // two records in flight: the next one is read from LDS while the
+            // current one is visited (ra and rb alternate, no copies)
+            F ra[NS], rb[NS];
+#pragma unroll
+            for (int v = 0; v < NS; v++) ra[v] = rec[(size_t) NS * l0 + v];
+            for (uint32_t l = l0;; l += 2) {
+              const bool hb = l + 1 < l1;
+              const uint32_t lb = hb ? l + 1 : l;
+#pragma unroll
+              for (int v = 0; v < NS; v++) rb[v] = rec[(size_t) NS * lb + v];
+              acc.visit(a, own, ra);
+              if (!hb) break;
+              const bool ha = l + 2 < l1;
+              const uint32_t la = ha ? l + 2 : l + 1;
+#pragma unroll
+              for (int v = 0; v < NS; v++) ra[v] = rec[(size_t) NS * la + v];
+              acc.visit(a, own, rb);
+              if (!ha) break;
+            }
+          }
         } else {
           const uint32_t g0 = T.global_of(r, l0);
           for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
@@ -312,10 +337,12 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   }
 }
 
-// Host side: LDS record capacity (twice the expected halo population, within
-// a 64 KiB block budget) and the block's dynamic LDS bytes.
+// Host side: LDS record capacity (`slack` times the expected halo population
+// plus 64, within a 64 KiB block budget) and the block's dynamic LDS bytes.
+// Less slack leaves room for more resident blocks per CU; a tile whose halo
+// does not fit reads from global memory instead (same results).
 template <class M, int TX, int TY, int TZ, int RM>
-inline void tile_budget(float occupancy, int *cap, size_t *smem) {
+inline void tile_budget(float occupancy, int *cap, size_t *smem, float slack = 2.0f) {
   using F = typename M::F;
   using TT = TileTables<M::DIM, TX, TY, TZ, RM>;
   const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
@@ -323,7 +350,7 @@ inline void tile_budget(float occupancy, int *cap, size_t *smem) {
                             (M::DIM == 3 ? (double) (TZ + 2 * RM) : 1.0);
   const size_t budget = 64 * 1024 - head;
   const int max_cap = (int) (budget / (M::NS * sizeof(F)));
-  const int want = (int) (2.0 * halo_cells * (double) occupancy) + 128;
+  const int want = (int) ((double) slack * halo_cells * (double) occupancy) + 64;
   *cap = want < max_cap ? want : max_cap;
   *smem = head + (size_t) *cap * M::NS * sizeof(F);
 }
@@ -338,9 +365,9 @@ inline unsigned tile_count(const GridView &g) {
 // Launch tiled_step for a tile shape.
 template <class M, int TX, int TY, int TZ, int RM>
 inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy, int block,
-                         hipStream_t stream) {
+                         hipStream_t stream, float slack = 2.0f) {
   size_t smem = 0;
-  tile_budget<M, TX, TY, TZ, RM>(occupancy, &a.cap, &smem);
+  tile_budget<M, TX, TY, TZ, RM>(occupancy, &a.cap, &smem, slack);
   const int threads = block > 256 ? 256 : block;
   const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
   hipLaunchKernelGGL((tiled_step<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a);

@@ -287,11 +287,16 @@ struct BoidPolicyC {
     p[1] = a.py[j];
   }
   struct Acc : BoidAccC {
+    float res[2];  // own record's rounding: (pos - org) - own, exact in double
     __device__ void visit(const Args &a, const float *own, const float *nb) {
-      BoidAccC::visit(a, nb[0] - own[0], nb[1] - own[1], nb[2], nb[3]);
+      BoidAccC::visit(a, (nb[0] - own[0]) - res[0], (nb[1] - own[1]) - res[1], nb[2], nb[3]);
     }
   };
-  __device__ static void begin(const Args &, Acc &, const float *) {}
+  __device__ static void begin(const Args &, Acc &s, const float *own, const double *p,
+                               const double *org) {
+    s.res[0] = (float) ((p[0] - org[0]) - (double) own[0]);
+    s.res[1] = (float) ((p[1] - org[1]) - (double) own[1]);
+  }
   __device__ static void finish(const Args &a, Acc &s, const float *own, uint32_t gi,
                                 const double *p) {
     boid_finish_c(a, s, p[0], p[1], own[2], own[3], gi);
@@ -300,9 +305,9 @@ struct BoidPolicyC {
 
 template <class M, int TX, int TY>
 void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
-  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream);
-  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream);
-  else launch_tiled<M, TX, TY, 1, 4>(a, L.grid, L.occupancy, L.block, L.stream);
+  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+  else launch_tiled<M, TX, TY, 1, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
 }
 
 void launch_c(const StepLaunch &L) {

@@ -224,14 +224,19 @@ struct CirclePolicyC {
     for (int d = 0; d < DIM; d++) p[d] = a.p[d][j];
   }
   struct Acc : CircleAccC<DIM> {
+    float res[DIM];  // own record's rounding: (pos - org) - own, exact in double
     __device__ void visit(const Args &a, const float *own, const float *nb) {
       float o[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-      for (int d = 0; d < DIM; d++) o[d] = nb[d] - own[d];
+      for (int d = 0; d < DIM; d++) o[d] = (nb[d] - own[d]) - res[d];
       CircleAccC<DIM>::visit(a, o);
     }
   };
-  __device__ static void begin(const Args &, Acc &, const float *) {}
+  __device__ static void begin(const Args &, Acc &s, const float *own, const double *p,
+                               const double *org) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.res[d] = (float) ((p[d] - org[d]) - (double) own[d]);
+  }
   __device__ static void finish(const Args &a, Acc &s, const float *, uint32_t i, const double *p) {
     circle_finish_c<DIM>(a, s, p, i);
   }
@@ -239,9 +244,9 @@ struct CirclePolicyC {
 
 template <class M, int TX, int TY, int TZ>
 void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
-  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream);
-  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream);
-  else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream);
+  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+  else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
 }
 
 template <int DIM>

@@ -97,9 +97,9 @@ void launch_typed(const StepLaunch &L) {
   a.n = L.n;
   using M = LifePolicy<F>;
   if (L.tile != 0 && L.grid.reach == 1) {
-    launch_tiled<M, 16, 8, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream);
+    launch_tiled<M, 16, 8, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
   } else if (L.tile != 0 && L.grid.reach == 2) {
-    launch_tiled<M, 16, 8, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream);
+    launch_tiled<M, 16, 8, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
   } else {
     hipLaunchKernelGGL(life_step<F>, grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   }

@@ -457,9 +457,14 @@ int init_typed(abl_sim *sim, const double *p) {
       m.py = host_random<F>((F) 0, env);
       m.px = host_random<F>((F) 0, env);
       const F phi = host_random<F>((F) 0, (F) (2.0 * pi));
-      // C's sin/cos take and return double, also under use_float
-      m.dx = (F) sin((double) phi);
-      m.dy = (F) cos((double) phi);
+      // C's sin/cos take and return double, also under use_float.  gcc -O2
+      // merges sin(phi) and cos(phi) into one sincos() call, whose result
+      // can differ from separate sin/cos calls in the last bit on some
+      // CPUs, so the same call is made here.
+      double s_phi, c_phi;
+      sincos((double) phi, &s_phi, &c_phi);
+      m.dx = (F) s_phi;
+      m.dy = (F) c_phi;
       m.sx = (F) 0;
       m.sy = (F) 0;
       m.life = gain + abl_random_int(0, 10);
