@@ -1,12 +1,14 @@
 """Launch-shape autotuner for the -b hip runtime.
 
 For one simulation configuration it times a few timesteps under each
-candidate runtime configuration -- threads per block ("hip.block") and cells
-per granularity ("hip.cell_scale", i.e. cell edge = granularity / scale with
-a (2*scale+1)^d stencil) -- on a private copy of the state, and returns the
+candidate runtime configuration -- threads per block ("hip.block"), cells per
+granularity ("hip.cell_scale", i.e. cell edge = granularity / scale with a
+(2*scale+1)^d stencil), LDS tile shape ("hip.tile", 0 = untiled) and LDS
+budget ("hip.lds_slack") -- on a private copy of the state, and returns the
 fastest.  Every candidate computes the same neighbour sets (the near test is
-exact whatever the cell size), so tuning never changes results beyond the
-summation order of the default cell-ordered visit.
+exact whatever the cell size), and tile shape and LDS budget do not change
+results at all, so tuning never changes results beyond the summation order
+of the default cell-ordered visit (which depends on the cell size).
 """
 from __future__ import annotations
 
@@ -17,8 +19,10 @@ from .runtime import Simulation
 
 
 DEFAULT_SPACE = {
-    "hip.block": [64, 128, 256, 512],
+    "hip.block": [128, 256],
     "hip.cell_scale": [1, 2, 3],
+    "hip.tile": [0, 1, 2, 3],
+    "hip.lds_slack": [1.25, 2.0],
 }
 
 
