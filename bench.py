@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -44,7 +45,8 @@ PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.
 # block): the fastest of a sweep over tile shapes 0-3, cell scales 1-3 and
 # blocks of 128/256 on this workload (openabl_amd/autotune.py runs the same
 # search for any program)
-CONFIG = {"hip.cell_scale": 2, "hip.tile": 1, "hip.block": 256, "hip.lds_slack": 1.25}
+CONFIG = {"hip.cell_scale": 2, "hip.tile": 1, "hip.block": 256, "hip.lds_slack": 1.25,
+          "hip.nlist": 1, "hip.skin": 0.05}
 
 
 def neighbour_bytes(n: int, precision: str, cells: int) -> int:
@@ -63,6 +65,12 @@ def kernel_name(precision: str) -> str:
                 "f32c": "boids_step_c<false>"}[precision]
     pol = {"f64": "BoidPolicy<double>", "f32": "BoidPolicy<float>", "f32c": "BoidPolicyC"}[precision]
     tx = {1: 16, 2: 8, 3: 32}[tile]
+    if CONFIG.get("hip.nlist", 1):
+        # stencil reach of the list radius (1 + skin) * r, in cells of r / cs
+        reach = math.ceil(cs * (1.0 + CONFIG.get("hip.skin", 0.05)) / (1.0 + 2.0 ** -20)
+                          * (1.0 + 2.0 ** -21))
+        rm = 1 if reach <= 1 else (2 if reach == 2 else 4)
+        return f"tiled_step_list<{pol}, {tx}, 8, 1, {rm}>"
     rm = 1 if cs <= 1 else (2 if cs == 2 else 4)
     return f"tiled_step<{pol}, {tx}, 8, 1, {rm}>"
 

@@ -79,6 +79,50 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double) s - (double) c; }
 };
 
+// Column pointers of one agent type for a fused copy src[slot] -> dst[slot'].
+struct GatherCols {
+  const void *src[kMaxCols];
+  void *dst[kMaxCols];
+  int elem[kMaxCols];
+  int ncols;
+};
+
+__device__ __forceinline__ void gather_cols(const GatherCols &g, uint32_t src, uint32_t dst) {
+  for (int c = 0; c < g.ncols; c++) {
+    switch (g.elem[c]) {
+      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
+      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
+      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
+    }
+  }
+}
+
+// Neighbour-list state of one agent type (abl_tile.hpp, tiled_list_build /
+// tiled_step_list).
+constexpr uint16_t kListOverflow = 0xFFFF;
+
+struct NListArgs {
+  uint16_t *idx;        // [n][cap] halo indices (cap a multiple of 8)
+  uint16_t *len;        // [n] entries, or kListOverflow
+  double *pos0;         // [dim][n] positions at the build
+  uint32_t *gate;       // [2] rebuild flags by timestep parity
+  uint32_t *stats;      // [0]: longest list at the last build
+  int n, cap, parity;
+  float rc_list;        // (radius + skin) in cell units, widened
+  double r2_list;       // (radius + skin)^2, widened
+  double half_skin2;    // (skin / 2)^2
+  // rebuild only: copy of the freshly placed agents back to the current buffers
+  GatherCols copy;
+  const int32_t *id_src;
+  int32_t *id_dst;
+  const uint32_t *skey_src;
+  uint32_t *skey_dst;
+  const uint32_t *start_src;
+  uint32_t *start_dst;
+  int ncells;
+  int gated;            // 1: build only when gate[parity] is set
+};
+
 // Fast float reciprocal (1 ulp) for the FP32-compensated precision.
 __device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
 

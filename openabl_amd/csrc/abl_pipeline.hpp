@@ -78,23 +78,6 @@ static __global__ void k_scatter(const uint32_t *__restrict__ key, uint32_t *__r
   tmp_id[slot] = id[i];
 }
 
-struct GatherCols {
-  const void *src[kMaxCols];
-  void *dst[kMaxCols];
-  int elem[kMaxCols];
-  int ncols;
-};
-
-__device__ __forceinline__ void gather_cols(const GatherCols &g, uint32_t src, uint32_t dst) {
-  for (int c = 0; c < g.ncols; c++) {
-    switch (g.elem[c]) {
-      case 1: ((uint8_t *) g.dst[c])[dst] = ((const uint8_t *) g.src[c])[src]; break;
-      case 4: ((uint32_t *) g.dst[c])[dst] = ((const uint32_t *) g.src[c])[src]; break;
-      default: ((uint64_t *) g.dst[c])[dst] = ((const uint64_t *) g.src[c])[src]; break;
-    }
-  }
-}
-
 // Stable placement inside each cell by id, fused with the column gather.
 // Optionally records the sorted keys (skey_out) and restores the per-cell
 // counts the scatter consumed (count_out, written by each cell's rank-0
@@ -140,7 +123,9 @@ static __global__ __launch_bounds__(kScanBlock) void k_tile_totals(const uint32_
 static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ count,
                                                                   const uint32_t *__restrict__ bsum,
                                                                   uint32_t *__restrict__ start,
-                                                                  int ncells) {
+                                                                  int ncells,
+                                                                  const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t lds[kWave + 1];
   uint32_t part = 0;
   for (int t = threadIdx.x; t < (int) blockIdx.x; t += blockDim.x) part += bsum[t];
@@ -188,7 +173,9 @@ static __global__ void k_bin_delta(const P *__restrict__ px, const P *__restrict
                                    const P *__restrict__ pz, GridView g,
                                    const uint32_t *__restrict__ skey, uint32_t *__restrict__ key,
                                    uint32_t *__restrict__ count, uint32_t *__restrict__ bsum,
-                                   MoverLists ml, uint32_t gen, int n) {
+                                   MoverLists ml, uint32_t gen, int n,
+                                   const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;  // neighbour-list mode: no rebuild this timestep
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int3 c = cell_of<P>(g, px[i], py[i], g.dim == 3 ? pz[i] : (P) 0);
@@ -216,10 +203,16 @@ static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t 
                                const uint32_t *__restrict__ start_old,
                                const uint32_t *__restrict__ start_new, MoverLists ml, uint32_t gen,
                                GatherCols g, const int32_t *__restrict__ id_in,
-                               int32_t *__restrict__ id_out, int n) {
+                               int32_t *__restrict__ id_out, int n,
+                               const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  // the counter the next generation uses starts from zero
-  if (s == 0) ml.nmov[(gen + 1u) & 1u] = 0u;
+  // the mover counters start from zero at the next build (k_bin_delta is done
+  // with this generation's counter)
+  if (s == 0) {
+    ml.nmov[0] = 0u;
+    ml.nmov[1] = 0u;
+  }
   if (s >= n) return;
   const uint32_t k = key[s];
   uint32_t dst;

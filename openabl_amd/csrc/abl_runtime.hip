@@ -130,6 +130,13 @@ struct AgentStore {
   bool nmov_clean = false;
   GridView sgv{};
   size_t sorted_n = 0;
+  // neighbour lists (abl_tile.hpp): valid for the stored order while nl_ok
+  bool nl_ok = false;
+  uint16_t *nl_idx = nullptr, *nl_len = nullptr;
+  double *nl_pos0 = nullptr;
+  uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list
+  int nl_cap = 0;               // entries per agent
+  size_t nl_ncap = 0;           // agents allocated
   // run-time add/remove: ids stop being 0..n-1 once agents die or are born
   int64_t next_id = 0;
   bool ids_dense = true;
@@ -152,6 +159,9 @@ struct abl_sim {
   int tile = 1;
   bool grid_delta = true;  // "hip.grid": delta re-sort when the stored order allows it
   float lds_slack = 2.0f;  // "hip.lds_slack": LDS tile records per expected halo agent
+  bool nlist = true;       // "hip.nlist": neighbour lists for steps that support them
+  double skin = 0.05;      // "hip.skin": list skin as a fraction of the near radius
+  int list_cap = 0;        // "hip.list_cap": entries per list (0: sized from the data)
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -227,6 +237,16 @@ static void free_store(AgentStore &s) {
   s.skey[0] = s.skey[1] = s.dirty = s.mslot = s.nmov = nullptr;
   s.mhead = s.mnext = nullptr;
   s.sorted = false;
+  void *nl_ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0, s.nl_gate};
+  for (void *p : nl_ptrs) {
+    if (p) (void) hipFree(p);
+  }
+  s.nl_idx = s.nl_len = nullptr;
+  s.nl_pos0 = nullptr;
+  s.nl_gate = nullptr;
+  s.nl_cap = 0;
+  s.nl_ncap = 0;
+  s.nl_ok = false;
   for (int c = 0; c < kMaxCols; c++) {
     if (s.bmem[c]) (void) hipFree(s.bmem[c]);
     s.bmem[c] = nullptr;
@@ -291,6 +311,7 @@ static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
   }
   s.cap = cap;
   s.sorted = false;
+  s.nl_ok = false;
   return ABL_OK;
 }
 
@@ -531,6 +552,7 @@ static int setup_grid(abl_sim *sim, AgentStore &s) {
     HIP_TRY(hipMemsetAsync(s.mhead, 0, (ncells + 1) * 8, sim->stream));
     s.cells_cap = ncells + 1;
     s.sorted = false;
+    s.nl_ok = false;
   }
   s.ncells = ncells;
   g.start = s.startb[s.start_cur];
@@ -546,7 +568,8 @@ static bool same_grid(const GridView &a, const GridView &b) {
 }
 
 template <typename P>
-static void launch_bin(abl_sim *sim, AgentStore &s, bool delta, int n) {
+static void launch_bin(abl_sim *sim, AgentStore &s, bool delta, int n,
+                       const uint32_t *gate = nullptr) {
   const int pc = s.pos_col;
   const P *px = (const P *) col_cur(s, pc);
   const P *py = (const P *) col_cur(s, pc + 1);
@@ -554,10 +577,32 @@ static void launch_bin(abl_sim *sim, AgentStore &s, bool delta, int n) {
   if (delta) {
     MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
     hipLaunchKernelGGL(k_bin_delta<P>, grid_blocks(n, 256), 256, 0, sim->stream, px, py, pz, s.gv,
-                       s.skey[s.skey_cur], s.key, s.count, s.bsum, ml, s.gen, n);
+                       s.skey[s.skey_cur], s.key, s.count, s.bsum, ml, s.gen, n, gate);
   } else {
     hipLaunchKernelGGL(k_bin<P>, grid_blocks(n, 256), 256, 0, sim->stream, px, py, pz, s.gv, s.key,
                        s.count, n);
+  }
+}
+
+// The delta re-sort's kernels into the other buffers (start_new, skey, ids,
+// columns gc.dst); with a gate they do nothing unless *gate is set.
+static void launch_delta(abl_sim *sim, AgentStore &s, const uint32_t *gate, uint32_t *start_old,
+                         uint32_t *start_new, const GatherCols &gc) {
+  const int n = (int) s.n;
+  const int nc = (int) s.ncells;
+  const int ntiles = (nc + kScanTile - 1) / kScanTile;
+  hipStream_t st = sim->stream;
+  if (n > 0) {
+    if (s.cols[s.pos_col].elem == 8) launch_bin<double>(sim, s, true, n, gate);
+    else launch_bin<float>(sim, s, true, n, gate);
+  }
+  hipLaunchKernelGGL(k_scan_apply, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum,
+                     start_new, nc, gate);
+  if (n > 0) {
+    MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
+    hipLaunchKernelGGL(k_place, grid_blocks(n, 256), 256, 0, st, s.key, s.skey[s.skey_cur],
+                       s.skey[1 - s.skey_cur], start_old, start_new, ml, s.gen, gc,
+                       s.id[s.id_cur], s.id[1 - s.id_cur], n, gate);
   }
 }
 
@@ -591,19 +636,8 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
   }
   if (delta) {
     if (!s.nmov_clean) HIP_TRY(hipMemsetAsync(s.nmov, 0, 2 * 4, st));
-    if (n > 0) {
-      if (dbl) launch_bin<double>(sim, s, true, n);
-      else launch_bin<float>(sim, s, true, n);
-    }
-    hipLaunchKernelGGL(k_scan_apply, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum,
-                       start_new, nc);
-    if (n > 0) {
-      MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
-      hipLaunchKernelGGL(k_place, grid_blocks(n, 256), 256, 0, st, s.key, s.skey[s.skey_cur],
-                         s.skey[1 - s.skey_cur], start_old, start_new, ml, s.gen, gc,
-                         s.id[s.id_cur], s.id[1 - s.id_cur], n);
-    }
-    s.nmov_clean = n > 0;
+    launch_delta(sim, s, nullptr, start_old, start_new, gc);
+    s.nmov_clean = true;
   } else {
     // the counts of the previous build are still in `count`
     HIP_TRY(hipMemsetAsync(s.count, 0, (size_t) (nc + 1) * 4, st));
@@ -633,6 +667,7 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
   s.gv.start = start_new;
   s.sgv = s.gv;
   s.sorted = true;
+  s.nl_ok = false;  // the stored order changed: lists index the old one
   s.sorted_n = s.n;
   s.grid_valid = true;
   return ABL_OK;
@@ -743,7 +778,10 @@ extern "C" int abl_hip_set_param(abl_sim *sim, const char *name, double value) {
   for (size_t p = 0; p < sim->prog->params.size(); p++) {
     if (std::strcmp(sim->prog->params[p].name, name) == 0) {
       sim->params[p] = value;
-      for (AgentStore &s : sim->types) s.grid_valid = false;
+      for (AgentStore &s : sim->types) {
+        s.grid_valid = false;
+        s.nl_ok = false;
+      }
       return ABL_OK;
     }
   }
@@ -766,6 +804,8 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
   CHECK_SIM(sim);
   if (!key || !value) return fail(ABL_ERR_ARG, "null argument");
   std::string k(key), v(value);
+  // launch settings decide the lists' tiles, radius and reach: rebuild them
+  for (AgentStore &s : sim->types) s.nl_ok = false;
   if (k == "hip.block") {
     int b = std::atoi(value);
     if (b < 64 || b > 1024 || b % 64) return fail(ABL_ERR_ARG, "hip.block must be 64..1024, a multiple of 64");
@@ -785,6 +825,18 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     int t = std::atoi(value);
     if (t < 0 || t > 3) return fail(ABL_ERR_ARG, "hip.tile must be 0..3");
     sim->tile = t;
+  } else if (k == "hip.nlist") {
+    if (v == "1" || v == "true") sim->nlist = true;
+    else if (v == "0" || v == "false") sim->nlist = false;
+    else return fail(ABL_ERR_ARG, "hip.nlist must be 0 or 1");
+  } else if (k == "hip.list_cap") {
+    const int c = std::atoi(value);
+    if (c < 0 || c > 65534) return fail(ABL_ERR_ARG, "hip.list_cap must be 0..65534");
+    sim->list_cap = c;
+  } else if (k == "hip.skin") {
+    const double f = std::atof(value);
+    if (!(f > 0.0 && f <= 1.0)) return fail(ABL_ERR_ARG, "hip.skin must be in (0, 1]");
+    sim->skin = f;
   } else if (k == "hip.lds_slack") {
     const double f = std::atof(value);
     if (!(f >= 1.0 && f <= 4.0)) return fail(ABL_ERR_ARG, "hip.lds_slack must be 1..4");
@@ -842,6 +894,7 @@ extern "C" int abl_hip_upload(abl_sim *sim, int t, const void *aos, size_t count
   s.n = count;
   s.grid_valid = false;
   s.sorted = false;
+  s.nl_ok = false;
   s.ids_dense = true;
   s.next_id = (int64_t) count;
   if (count == 0) return ABL_OK;
@@ -1039,6 +1092,7 @@ static int apply_removals(abl_sim *sim, AgentStore &a, int n) {
   a.ids_dense = false;
   a.grid_valid = false;
   a.sorted = false;
+  a.nl_ok = false;
   return ABL_OK;
 }
 
@@ -1066,6 +1120,130 @@ static int apply_births(abl_sim *sim, AgentStore &parent, AgentStore &child, int
   child.ids_dense = false;
   child.grid_valid = false;
   child.sorted = false;
+  child.nl_ok = false;
+  return ABL_OK;
+}
+
+// ---- neighbour lists ---------------------------------------------------------------
+// A step whose near loop runs over its own agent type, with a launcher that
+// supports them, walks per-agent neighbour lists (abl_tile.hpp) instead of
+// the stencil.  The lists are rebuilt on the device -- delta re-sort into the
+// other buffers, list build, copy back -- on timesteps whose gate the
+// previous step set, so the host never waits and its buffer bookkeeping does
+// not depend on whether a rebuild ran.
+
+static bool list_eligible(const abl_sim *sim, const StepDesc &st, const AgentStore &a) {
+  return sim->nlist && st.nlist && st.near_agent == st.agent && sim->order == 0 && sim->tile != 0 &&
+         !st.removes && !st.adds && a.n > 0 && a.pos_col >= 0 && a.n < (size_t) INT32_MAX;
+}
+
+static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
+  cap = (cap + 7) & ~7;  // whole 16-byte rows of entries
+  if (cap > 65528) cap = 65528;
+  if (s.nl_idx && s.nl_cap >= cap && s.nl_ncap >= s.n) return ABL_OK;
+  void *ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0};
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  for (void *p : ptrs) {
+    if (p) HIP_TRY(hipFree(p));
+  }
+  const size_t n = std::max(s.n, s.cap);
+  HIP_TRY(hipMalloc(&s.nl_idx, (size_t) cap * n * 2));
+  HIP_TRY(hipMalloc(&s.nl_len, n * 2));
+  HIP_TRY(hipMalloc(&s.nl_pos0, 3 * n * 8));
+  if (!s.nl_gate) HIP_TRY(hipMalloc(&s.nl_gate, 4 * 4));
+  s.nl_cap = cap;
+  s.nl_ncap = n;
+  return ABL_OK;
+}
+
+static NListArgs list_args(const abl_sim *sim, const AgentStore &s, double r, double rl) {
+  NListArgs nl{};
+  nl.idx = s.nl_idx;
+  nl.len = s.nl_len;
+  nl.pos0 = s.nl_pos0;
+  nl.gate = s.nl_gate;
+  nl.stats = s.nl_gate + 2;
+  nl.n = (int) s.n;
+  nl.cap = s.nl_cap;
+  nl.parity = (int) (sim->timestep & 1u);
+  nl.rc_list = (float) (rl / s.cell * (1.0 + 1.0 / 1048576.0));
+  // widened far past the float error of the records (relative records: ~1e-6)
+  const double rw = rl * (1.0 + 1e-6) + 1e-5 * r;
+  nl.r2_list = rw * rw;
+  const double h = 0.5 * (rl - r);
+  nl.half_skin2 = h * h;
+  nl.ncells = (int) s.ncells;
+  return nl;
+}
+
+// Enter list mode on a fresh grid: build the lists from the current buffers,
+// grow them once if the longest list did not fit (one host wait per entry).
+static int enter_lists(abl_sim *sim, AgentStore &b, const StepDesc &st, const StepLaunch &Lb,
+                       double r, double rl) {
+  const double vol = b.dim == 3 ? 4.0 / 3.0 * M_PI * rl * rl * rl : M_PI * rl * rl;
+  const double env = (double) b.ncells * std::pow(b.cell, (double) b.dim);
+  const double expect = env > 0 ? vol * (double) b.n / env : 0.0;
+  const int want = sim->list_cap ? sim->list_cap : std::max(b.nl_cap, (int) (1.5 * expect) + 16);
+  if (sim->list_cap && b.nl_cap != ((sim->list_cap + 7) & ~7)) b.nl_cap = 0;  // size requested
+  int rc = reserve_lists(sim, b, want);
+  if (rc) return rc;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    HIP_TRY(hipMemsetAsync(b.nl_gate, 0, 4 * 4, sim->stream));
+    NListArgs nl = list_args(sim, b, r, rl);
+    StepLaunch L = Lb;
+    L.nl_mode = 1;
+    L.nl = &nl;
+    st.launch(L);
+    HIP_TRY(hipGetLastError());
+    uint32_t longest = 0;
+    HIP_TRY(hipMemcpyAsync(&longest, b.nl_gate + 2, 4, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    if ((int) longest <= b.nl_cap || b.nl_cap >= 65528 || sim->list_cap) break;
+    rc = reserve_lists(sim, b, (int) (1.25 * longest) + 16);
+    if (rc) return rc;
+  }
+  b.nl_ok = true;
+  return ABL_OK;
+}
+
+// The gated rebuild of one timestep: re-sort into the other buffers, build
+// the lists there and copy the agents back, or nothing when the gate is clear.
+static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const StepLaunch &Lb,
+                         double r, double rl) {
+  b.gen += 1;
+  if (b.gen == 0) b.gen = 1;
+  const uint32_t *gate = b.nl_gate + (sim->timestep & 1u);
+  uint32_t *start_cur = b.startb[b.start_cur];
+  uint32_t *start_other = b.startb[1 - b.start_cur];
+  GatherCols placed{};
+  placed.ncols = (int) b.cols.size();
+  for (int c = 0; c < placed.ncols; c++) {
+    placed.src[c] = col_cur(b, c);
+    placed.dst[c] = col_other(b, c);
+    placed.elem[c] = b.cols[c].elem;
+  }
+  launch_delta(sim, b, gate, start_cur, start_other, placed);
+  NListArgs nl = list_args(sim, b, r, rl);
+  nl.gated = 1;
+  nl.copy.ncols = placed.ncols;
+  for (int c = 0; c < placed.ncols; c++) {
+    nl.copy.src[c] = placed.dst[c];
+    nl.copy.dst[c] = (void *) placed.src[c];
+    nl.copy.elem[c] = placed.elem[c];
+  }
+  nl.id_src = b.id[1 - b.id_cur];
+  nl.id_dst = b.id[b.id_cur];
+  nl.skey_src = b.skey[1 - b.skey_cur];
+  nl.skey_dst = b.skey[b.skey_cur];
+  nl.start_src = start_other;
+  nl.start_dst = start_cur;
+  StepLaunch L = Lb;
+  for (size_t c = 0; c < b.cols.size(); c++) L.in.c[c] = col_other(b, (int) c);
+  L.grid.start = start_other;
+  L.nl_mode = 1;
+  L.nl = &nl;
+  st.launch(L);
+  HIP_TRY(hipGetLastError());
   return ABL_OK;
 }
 
@@ -1093,9 +1271,14 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
     L.in.c[c] = col_cur(a, (int) c);
     L.out.c[c] = (st.writes >> c) & 1u ? col_other(a, (int) c) : nullptr;
   }
+  NListArgs nl_step{};
   if (st.near_agent >= 0) {
     AgentStore &b = sim->types[st.near_agent];
-    if (!b.grid_valid) {
+    const double r = step_radius(sim, st);
+    const double rl = r * (1.0 + sim->skin);
+    const bool want_list = list_eligible(sim, st, a);
+    const bool gated = want_list && b.nl_ok && reach_for(b, rl) <= 4;
+    if (!gated && !b.grid_valid) {
       int sp = span_open(sim, 2);
       int rc = build_grid(sim, b);
       span_close(sim, sp);
@@ -1107,7 +1290,6 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
         }
       }
     }
-    const double r = step_radius(sim, st);
     L.nb_n = (int) b.n;
     for (size_t c = 0; c < b.cols.size(); c++) L.nb.c[c] = col_cur(b, (int) c);
     L.nb_id = b.id[b.id_cur];
@@ -1118,6 +1300,19 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
     L.lds_slack = sim->lds_slack;
     L.occupancy = b.ncells ? (float) ((double) b.n / (double) b.ncells) : 0.f;
     L.rc = (float) (r / b.cell * (1.0 + 1.0 / 1048576.0));
+    // lists need the list radius inside the largest tile halo (4 cells)
+    if (want_list && reach_for(b, rl) <= 4) {
+      L.grid.reach = reach_for(b, rl);
+      int sp = span_open(sim, 2);
+      int rc = gated ? gated_rebuild(sim, b, st, L, r, rl) : enter_lists(sim, b, st, L, r, rl);
+      span_close(sim, sp);
+      if (rc) return rc;
+      nl_step = list_args(sim, b, r, rl);
+      L.nl_mode = 2;
+      L.nl = &nl_step;
+    } else {
+      b.nl_ok = false;
+    }
     if (sim->order == 1) {
       int side = 2 * L.grid.reach + 1;
       int runs = side * side * (b.dim == 3 ? side : 1);

@@ -74,6 +74,7 @@ struct StepDesc {
   void (*launch)(const StepLaunch &);
   bool removes;              // calls removeCurrent()
   int adds;                  // 0, or 1 + the agent type it add()s at run time
+  bool nlist;                // the launcher supports neighbour lists (StepLaunch::nl_mode)
 };
 
 struct ProgramDesc {
@@ -119,6 +120,9 @@ struct StepLaunch {
   int tile;            // LDS tile variant (0: untiled thread-per-agent kernel)
   float occupancy;     // near-type agents per cell (sizes the LDS tile budget)
   float lds_slack;     // LDS records per expected halo agent
+  // neighbour lists: 0 = stencil walk, 1 = build the lists, 2 = step through them
+  int nl_mode;
+  const NListArgs *nl;
   float rc;            // near radius in cell units, widened by 2^-20
   // run-time add()/removeCurrent() and random() inside steps
   const int32_t *self_id;  // ids of the stepping agents

@@ -19,6 +19,7 @@
 #pragma once
 
 #include "abl_device.hpp"
+#include "abl_runtime.hpp"
 
 namespace abl {
 
@@ -85,7 +86,8 @@ struct TileTables {
       lbase[0] = 0;
       for (int r = 0; r < nrows; r++) lbase[r + 1] += lbase[r];
       total = (int) lbase[nrows];
-      overflow = total > cap ? 1 : 0;
+      // one record more than the halo: the far-away pad record of the lists
+      overflow = total >= cap ? 1 : 0;
     }
     __syncthreads();
     const int w = hn[0] + 1;
@@ -227,114 +229,338 @@ __global__ __launch_bounds__(256) void tiled_step_multi(const typename M::Args *
   tiled_body<M, TX, TY, TZ, RM>(args[lo], t - tile_base[lo], smem);
 }
 
+// A coordinate whose square overflows to +inf in F.
+template <typename F>
+constexpr double kFarAway = sizeof(F) == 8 ? 1e300 : 1e30;
+
+// One tile's block state: tables, LDS records, overflow, relative origin.
+template <class M, int TX, int TY, int TZ, int RM>
+struct TileCtx {
+  using F = typename M::F;
+  using TT = TileTables<M::DIM, TX, TY, TZ, RM>;
+  TT *T;
+  F *rec;
+  bool ovf;
+  double org[3];
+};
+
+// Set up tile `tile` and stage its halo records (block-cooperative).
+template <class M, int TX, int TY, int TZ, int RM>
+__device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename M::Args &a,
+                                                                 unsigned tile,
+                                                                 unsigned char *smem) {
+  using Ctx = TileCtx<M, TX, TY, TZ, RM>;
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  Ctx c;
+  c.T = reinterpret_cast<typename Ctx::TT *>(smem);
+  c.rec = reinterpret_cast<F *>(smem + ((sizeof(typename Ctx::TT) + 15) & ~(size_t) 15));
+  const GridView &g = a.g;
+  c.T->setup(g, g.reach, tile, a.cap);
+  c.ovf = c.T->overflow != 0;
+  c.org[0] = c.org[1] = c.org[2] = 0.0;
+  if constexpr (policy_relative<M>::value) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+      c.org[d] = g.lo[d] + ((double) c.T->t0[d] + 0.5 * (double) c.T->tn[d]) / g.inv_cell;
+  }
+  if (!c.ovf) {
+    for (int k = threadIdx.x; k < c.T->total; k += blockDim.x) {
+      const uint32_t j = c.T->global_of(c.T->row_of_record((uint32_t) k), (uint32_t) k);
+      if constexpr (policy_relative<M>::value) M::load(a, j, c.rec + (size_t) NS * k, c.org);
+      else M::load(a, j, c.rec + (size_t) NS * k);
+    }
+    // record `total`: a position no near test accepts (its squares overflow),
+    // the entry neighbour lists are padded with
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int v = 0; v < NS; v++)
+        c.rec[(size_t) NS * c.T->total + v] = v < DIM ? (F) kFarAway<F> : (F) 0;
+    }
+  }
+  __syncthreads();
+  return c;
+}
+
+template <class M, class Ctx>
+__device__ __forceinline__ void tile_load(const typename M::Args &a, const Ctx &c, uint32_t j,
+                                          typename M::F *r) {
+  if constexpr (policy_relative<M>::value) M::load(a, j, r, c.org);
+  else M::load(a, j, r);
+}
+
+// Own agent k of the tile: global slot, record, FP64 position, cell and
+// fractional in-cell position.
+template <class M, class Ctx>
+__device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ctx &c, int k,
+                                             typename M::F *own, double *pd, int3 *cell,
+                                             float *u) {
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  const GridView &g = a.g;
+  int orow;
+  uint32_t ol;
+  c.T->own(k, &orow, &ol);
+  const uint32_t gi = c.T->global_of(orow, ol);
+  if (!c.ovf) {
+#pragma unroll
+    for (int v = 0; v < NS; v++) own[v] = c.rec[(size_t) NS * ol + v];
+  } else {
+    tile_load<M>(a, c, gi, own);
+  }
+  // the agent's cell comes from its stored position, exactly as binned
+  pd[0] = pd[1] = pd[2] = 0.0;
+  if constexpr (policy_relative<M>::value) {
+    M::position(a, gi, pd);
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
+  }
+  *cell = cell_of<double>(g, pd[0], pd[1], pd[2]);
+  u[0] = (float) ((pd[0] - g.lo[0]) * g.inv_cell - (double) cell->x);
+  u[1] = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) cell->y);
+  u[2] = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) cell->z) : 0.f;
+  return gi;
+}
+
+// Walk the (narrowed) stencil of radius rc cell units around an own agent:
+// fn(nb, l) for every candidate record nb, l its halo index, row by row of
+// the stencil, each row in cell-sorted order.
+template <class M, bool OVF, class Ctx, class Fn>
+__device__ __forceinline__ void tile_walk_src(const typename M::Args &a, const Ctx &c, int3 cell,
+                                              const float *u, float rc, Fn &fn) {
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  const auto &T = *c.T;
+  const int R = a.g.reach;
+  const int zr = DIM == 3 ? R : 0;
+  for (int dz = -zr; dz <= zr; dz++) {
+    const int zz = cell.z + dz;
+    if (zz < T.h0[2] || zz >= T.h0[2] + T.hn[2]) continue;
+    for (int dy = -R; dy <= R; dy++) {
+      const int yy = cell.y + dy;
+      if (yy < T.h0[1] || yy >= T.h0[1] + T.hn[1]) continue;
+      int xa, xb;
+      if (!row_interval(u[0], u[1], u[2], dy, dz, rc, R, &xa, &xb)) continue;
+      const int xlo = max(cell.x + xa, T.h0[0]);
+      const int xhi = min(cell.x + xb, T.h0[0] + T.hn[0] - 1);
+      if (xlo > xhi) continue;
+      const int r = (zz - T.h0[2]) * T.hn[1] + (yy - T.h0[1]);
+      const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
+      const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
+      if constexpr (!OVF) {
+        if (l0 < l1) {
+          // two records in flight: the next one is read from LDS while the
+          // current one is visited (ra and rb alternate, no copies)
+          F ra[NS], rb[NS];
+#pragma unroll
+          for (int v = 0; v < NS; v++) ra[v] = c.rec[(size_t) NS * l0 + v];
+          for (uint32_t l = l0;; l += 2) {
+            const bool hb = l + 1 < l1;
+            const uint32_t lb = hb ? l + 1 : l;
+#pragma unroll
+            for (int v = 0; v < NS; v++) rb[v] = c.rec[(size_t) NS * lb + v];
+            fn(ra, l);
+            if (!hb) break;
+            const bool ha = l + 2 < l1;
+            const uint32_t la = ha ? l + 2 : l + 1;
+#pragma unroll
+            for (int v = 0; v < NS; v++) ra[v] = c.rec[(size_t) NS * la + v];
+            fn(rb, l + 1);
+            if (!ha) break;
+          }
+        }
+      } else {
+        const uint32_t g0 = T.global_of(r, l0);
+        for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
+          F nb[NS];
+          tile_load<M>(a, c, j, nb);
+          fn(nb, l0 + (j - g0));
+        }
+      }
+    }
+  }
+}
+
+// The tile's records are in LDS or (overflow) in global memory for the whole
+// block: branch once, so each loop body is specialised for its source.
+template <class M, class Ctx, class Fn>
+__device__ __forceinline__ void tile_walk(const typename M::Args &a, const Ctx &c, int3 cell,
+                                          const float *u, float rc, Fn &&fn) {
+  if (!c.ovf) tile_walk_src<M, false>(a, c, cell, u, rc, fn);
+  else tile_walk_src<M, true>(a, c, cell, u, rc, fn);
+}
+
+// Visit the records of one agent's neighbour list: eight entries per 16-byte
+// load, the next eight loaded while these are visited.
+template <class M, bool OVF, class Ctx, class Fn>
+__device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &c,
+                                          const NListArgs &nl, uint32_t gi, uint32_t len, Fn &fn) {
+  using F = typename M::F;
+  constexpr int NS = M::NS;
+  const uint4 *lp = reinterpret_cast<const uint4 *>(nl.idx + (size_t) gi * nl.cap);
+  uint4 cur = len > 0 ? lp[0] : make_uint4(0, 0, 0, 0);
+  for (uint32_t e = 0; e < len; e += 8) {
+    const uint4 nxt = e + 8 < len ? lp[(e >> 3) + 1] : cur;
+    const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t l = (w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+      if constexpr (!OVF) {
+        // lists are padded to whole rows of eight with the far-away record
+        fn(c.rec + (size_t) NS * l);
+      } else if (l < (uint32_t) c.T->total) {
+        F nb[NS];
+        tile_load<M>(a, c, c.T->global_of(c.T->row_of_record(l), l), nb);
+        fn(nb);
+      }
+    }
+    cur = nxt;
+  }
+}
+
 template <class M, int TX, int TY, int TZ, int RM>
 __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned tile,
                                            unsigned char *smem) {
   using F = typename M::F;
-  constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  constexpr bool REL = policy_relative<M>::value;
-  using TT = TileTables<DIM, TX, TY, TZ, RM>;
-  TT &T = *reinterpret_cast<TT *>(smem);
-  F *rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
-  const GridView &g = a.g;
-  const int R = g.reach;
-  T.setup(g, R, tile, a.cap);
-  const bool ovf = T.overflow != 0;
-  // origin of relative records: the centre of the tile's own cells
-  double org[3] = {0.0, 0.0, 0.0};
-  if constexpr (REL) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++)
-      org[d] = g.lo[d] + ((double) T.t0[d] + 0.5 * (double) T.tn[d]) / g.inv_cell;
-  }
-  auto load = [&](uint32_t j, F *r) {
-    if constexpr (REL) M::load(a, j, r, org);
-    else M::load(a, j, r);
-  };
-  if (!ovf) {
-    for (int k = threadIdx.x; k < T.total; k += blockDim.x) {
-      load(T.global_of(T.row_of_record((uint32_t) k), (uint32_t) k), rec + (size_t) NS * k);
-    }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < T.own_total; k += blockDim.x) {
-    int orow;
-    uint32_t ol;
-    T.own(k, &orow, &ol);
-    const uint32_t gi = T.global_of(orow, ol);
+  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
+  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
     F own[NS];
-    if (!ovf) {
-#pragma unroll
-      for (int v = 0; v < NS; v++) own[v] = rec[(size_t) NS * ol + v];
-    } else {
-      load(gi, own);
-    }
-    // the agent's cell comes from its stored position, exactly as binned
-    double pd[3] = {0.0, 0.0, 0.0};
-    if constexpr (REL) {
-      M::position(a, gi, pd);
-    } else {
-#pragma unroll
-      for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
-    }
-    const int3 c = cell_of<double>(g, pd[0], pd[1], pd[2]);
-    const float ux = (float) ((pd[0] - g.lo[0]) * g.inv_cell - (double) c.x);
-    const float uy = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) c.y);
-    const float uz = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) c.z) : 0.f;
+    double pd[3];
+    int3 cell;
+    float u[3];
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
     typename M::Acc acc;
-    if constexpr (REL) M::begin(a, acc, own, pd, org);
+    if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
     else M::begin(a, acc, own);
-    const int zr = DIM == 3 ? R : 0;
-    for (int dz = -zr; dz <= zr; dz++) {
-      const int zz = c.z + dz;
-      if (zz < T.h0[2] || zz >= T.h0[2] + T.hn[2]) continue;
-      for (int dy = -R; dy <= R; dy++) {
-        const int yy = c.y + dy;
-        if (yy < T.h0[1] || yy >= T.h0[1] + T.hn[1]) continue;
-        int xa, xb;
-        if (!row_interval(ux, uy, uz, dy, dz, a.rc, R, &xa, &xb)) continue;
-        const int xlo = max(c.x + xa, T.h0[0]);
-        const int xhi = min(c.x + xb, T.h0[0] + T.hn[0] - 1);
-        if (xlo > xhi) continue;
-        const int r = (zz - T.h0[2]) * T.hn[1] + (yy - T.h0[1]);
-        const uint32_t l0 = T.cell_lds(r, xlo - T.h0[0]);
-        const uint32_t l1 = T.cell_lds(r, xhi - T.h0[0] + 1);
-        if (!ovf) {
-          if (l0 < l1) {
-            // two records in flight: the next one is read from LDS while the
-            // current one is visited (ra and rb alternate, no copies)
-            F ra[NS], rb[NS];
-#pragma unroll
-            for (int v = 0; v < NS; v++) ra[v] = rec[(size_t) NS * l0 + v];
-            for (uint32_t l = l0;; l += 2) {
-              const bool hb = l + 1 < l1;
-              const uint32_t lb = hb ? l + 1 : l;
-#pragma unroll
-              for (int v = 0; v < NS; v++) rb[v] = rec[(size_t) NS * lb + v];
-              acc.visit(a, own, ra);
-              if (!hb) break;
-              const bool ha = l + 2 < l1;
-              const uint32_t la = ha ? l + 2 : l + 1;
-#pragma unroll
-              for (int v = 0; v < NS; v++) ra[v] = rec[(size_t) NS * la + v];
-              acc.visit(a, own, rb);
-              if (!ha) break;
-            }
-          }
-        } else {
-          const uint32_t g0 = T.global_of(r, l0);
-          for (uint32_t j = g0; j < g0 + (l1 - l0); j++) {
-            F nb[NS];
-            load(j, nb);
-            acc.visit(a, own, nb);
-          }
-        }
-      }
-    }
-    if constexpr (REL) M::finish(a, acc, own, gi, pd);
+    tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
+    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd);
     else M::finish(a, acc, own, gi);
   }
+}
+
+// ---- neighbour lists -------------------------------------------------------------
+// Per-agent lists of the halo records within radius + skin, built by the
+// stencil walk right after a (re-)sort and reused while the stored order is
+// left alone: as long as no agent has moved more than skin / 2 since the
+// build, every pair within the radius now was within radius + skin then, so
+// walking the list (and applying the exact near test) finds exactly the
+// stencil's neighbours, in the stencil's order of the build.
+// Entries are halo indices of the build's tile (uint16), stored transposed
+// (entry k of agent i at idx[k * n + i]) so a wave reads them coalesced.
+// The step kernel flags a rebuild (gate of the next timestep) when an agent
+// strays past skin / 2 or a list overflowed; the rebuild kernels of the next
+// timestep run only when their gate is set.
+// Build the lists of one tile (after a sort; records come from the sorted
+// buffers a points at).
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NListArgs nl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (nl.gated && nl.gate[nl.parity] == 0u) return;
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
+  // the new cell offsets become the current ones (block-strided share)
+  if (nl.start_dst) {
+    for (int k = (int) (blockIdx.x * blockDim.x + threadIdx.x); k <= nl.ncells;
+         k += (int) (gridDim.x * blockDim.x))
+      nl.start_dst[k] = nl.start_src[k];
+  }
+  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
+  uint32_t longest = 0;
+  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
+    F own[NS];
+    double pd[3];
+    int3 cell;
+    float u[3];
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
+    uint32_t cnt = 0;
+    tile_walk<M>(a, c, cell, u, nl.rc_list, [&](const F *nb, uint32_t l) {
+      double d2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const double e = (double) nb[d] - (double) own[d];
+        d2 += e * e;
+      }
+      if (d2 <= nl.r2_list) {
+        if (cnt < (uint32_t) nl.cap) nl.idx[(size_t) gi * nl.cap + cnt] = (uint16_t) l;
+        cnt++;
+      }
+    });
+    const bool fits = cnt <= (uint32_t) nl.cap && cnt < kListOverflow;
+    nl.len[gi] = fits ? (uint16_t) cnt : kListOverflow;
+    if (fits) {
+      // pad to a whole row of eight with the far-away record (index total)
+      for (uint32_t e = cnt; e < ((cnt + 7u) & ~7u); e++)
+        nl.idx[(size_t) gi * nl.cap + e] = (uint16_t) c.T->total;
+    }
+    longest = max(longest, cnt);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) nl.pos0[(size_t) d * nl.n + gi] = pd[d];
+    if (nl.copy.ncols) {
+      gather_cols(nl.copy, gi, gi);
+      nl.id_dst[gi] = nl.id_src[gi];
+      nl.skey_dst[gi] = nl.skey_src[gi];
+    }
+  }
+  // one atomic per block for the longest list
+  __shared__ uint32_t blk_longest;
+  if (threadIdx.x == 0) blk_longest = 0u;
+  __syncthreads();
+  if (longest) atomicMax(&blk_longest, longest);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_longest) atomicMax(&nl.stats[0], blk_longest);
+}
+
+// One timestep of the step through the lists.  An agent whose list
+// overflowed walks the stencil (the grid is fresh: an overflow always
+// forces a rebuild on the next timestep).
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_step_list(typename M::Args a, NListArgs nl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
+  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
+  bool stray = false;
+  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
+    F own[NS];
+    double pd[3];
+    int3 cell;
+    float u[3];
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
+    typename M::Acc acc;
+    if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
+    else M::begin(a, acc, own);
+    const uint32_t len = nl.len[gi];
+    if (len != kListOverflow) {
+      auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
+      if (!c.ovf) list_walk<M, false>(a, c, nl, gi, len, fn);
+      else list_walk<M, true>(a, c, nl, gi, len, fn);
+    } else {
+      stray = true;
+      tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
+    }
+    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd);
+    else M::finish(a, acc, own, gi);
+    double np[3] = {0.0, 0.0, 0.0};
+    M::new_position(a, gi, np);
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const double e = np[d] - nl.pos0[(size_t) d * nl.n + gi];
+      d2 += e * e;
+    }
+    // !(d2 <= bound) also catches NaN positions
+    if (!(d2 <= nl.half_skin2)) stray = true;
+  }
+  if (stray) nl.gate[nl.parity ^ 1] = 1u;
 }
 
 // Host side: LDS record capacity (`slack` times the expected halo population
@@ -371,6 +597,37 @@ inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy,
   const int threads = block > 256 ? 256 : block;
   const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
   hipLaunchKernelGGL((tiled_step<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a);
+}
+
+// Launch the neighbour-list build (mode 1) or the step through the lists
+// (mode 2) for a tile shape; same tiles and LDS budget as tiled_step.
+template <class M, int TX, int TY, int TZ, int RM>
+inline void launch_list(typename M::Args a, const GridView &g, float occupancy, int block,
+                        hipStream_t stream, float slack, const NListArgs &nl, int mode) {
+  size_t smem = 0;
+  tile_budget<M, TX, TY, TZ, RM>(occupancy, &a.cap, &smem, slack);
+  const int threads = block > 256 ? 256 : block;
+  const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
+  if (mode == 1) {
+    hipLaunchKernelGGL((tiled_list_build<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a, nl);
+  } else {
+    hipLaunchKernelGGL((tiled_step_list<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a, nl);
+  }
+}
+
+// Dispatch on the stencil reach (RM = 1, 2 or 4 cells) and on the list mode.
+template <class M, int TX, int TY, int TZ>
+inline void launch_tiles(const StepLaunch &L, const typename M::Args &a) {
+  const int R = L.grid.reach;
+  if (L.nl_mode) {
+    if (R <= 1) launch_list<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
+    else if (R == 2) launch_list<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
+    else launch_list<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
+  } else {
+    if (R <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+    else if (R == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+    else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
+  }
 }
 
 }  // namespace abl

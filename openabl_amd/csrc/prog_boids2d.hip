@@ -159,6 +159,10 @@ struct BoidPolicy {
   __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t gi) {
     boid_finish<F>(a, s, own[0], own[1], own[2], own[3], gi);
   }
+  __device__ static void new_position(const Args &a, uint32_t gi, double *p) {
+    p[0] = (double) a.opx[gi];
+    p[1] = (double) a.opy[gi];
+  }
 };
 
 // Square-sum thresholds: near_t = max{f : sqrtf(f) <= interaction_radius};
@@ -301,14 +305,12 @@ struct BoidPolicyC {
                                 const double *p) {
     boid_finish_c(a, s, p[0], p[1], own[2], own[3], gi);
   }
+  __device__ static void new_position(const Args &a, uint32_t gi, double *p) {
+    p[0] = a.opx[gi];
+    p[1] = a.opy[gi];
+  }
 };
 
-template <class M, int TX, int TY>
-void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
-  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, 1, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, 1, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-  else launch_tiled<M, TX, TY, 1, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-}
 
 void launch_c(const StepLaunch &L) {
   BoidArgsC a{};
@@ -337,11 +339,11 @@ void launch_c(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((boids_step_c<true>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   } else if (L.tile == 1 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicyC, 16, 8>(L, a);
+    launch_tiles<BoidPolicyC, 16, 8, 1>(L, a);
   } else if (L.tile == 2 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicyC, 8, 8>(L, a);
+    launch_tiles<BoidPolicyC, 8, 8, 1>(L, a);
   } else if (L.tile == 3 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicyC, 32, 8>(L, a);
+    launch_tiles<BoidPolicyC, 32, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((boids_step_c<false>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   }
@@ -374,11 +376,11 @@ void launch_typed(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((boids_step<F, true>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   } else if (L.tile == 1 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicy<F>, 16, 8>(L, a);
+    launch_tiles<BoidPolicy<F>, 16, 8, 1>(L, a);
   } else if (L.tile == 2 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicy<F>, 8, 8>(L, a);
+    launch_tiles<BoidPolicy<F>, 8, 8, 1>(L, a);
   } else if (L.tile == 3 && L.grid.reach <= 4) {
-    launch_tile_r<BoidPolicy<F>, 32, 8>(L, a);
+    launch_tiles<BoidPolicy<F>, 32, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((boids_step<F, false>), grid_blocks(L.n, L.block), L.block, 0, L.stream, a);
   }
@@ -553,7 +555,7 @@ const ProgramDesc *program_boids2d() {
        {"time_scale", 0.0005}, {"global_scale", 0.15}, {"steer_scale", 1.0},
        {"collision_scale", 1.5}, {"match_scale", 1.0}, {"max_speed", 2.0},
        {"num_agents", 65536}, {"num_timesteps", 200}, {"min_pos", 0.0}, {"max_pos", 1000.0}},
-      {{"update_boid", 0, 0, "interaction_radius", 1.0, 0xfu, launch_update_boid}},
+      {{"update_boid", 0, 0, "interaction_radius", 1.0, 0xfu, launch_update_boid, false, 0, true}},
       env,
       host_init,
       nullptr,

@@ -133,6 +133,10 @@ struct CirclePolicy {
   __device__ static void finish(const Args &a, Acc &s, const F *, uint32_t i) {
     circle_finish<F, DIM>(a, s.np, i);
   }
+  __device__ static void new_position(const Args &a, uint32_t i, double *p) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = (double) a.o[d][i];
+  }
 };
 
 // ---- FP32-compensated precision (ABL_PREC_F32C) -----------------------------
@@ -240,14 +244,12 @@ struct CirclePolicyC {
   __device__ static void finish(const Args &a, Acc &s, const float *, uint32_t i, const double *p) {
     circle_finish_c<DIM>(a, s, p, i);
   }
+  __device__ static void new_position(const Args &a, uint32_t i, double *p) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = a.o[d][i];
+  }
 };
 
-template <class M, int TX, int TY, int TZ>
-void launch_tile_r(const StepLaunch &L, const typename M::Args &a) {
-  if (L.grid.reach <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-  else if (L.grid.reach == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-  else launch_tiled<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
-}
 
 template <int DIM>
 void launch_c(const StepLaunch &L) {
@@ -271,10 +273,10 @@ void launch_c(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((circle_step_c<DIM, true>), nb, L.block, 0, L.stream, a);
   } else if (L.tile != 0 && L.grid.reach <= 4) {
-    if (DIM == 3) launch_tile_r<M, 8, 4, 4>(L, a);
-    else if (L.tile == 2) launch_tile_r<M, 8, 8, 1>(L, a);
-    else if (L.tile == 3) launch_tile_r<M, 32, 8, 1>(L, a);
-    else launch_tile_r<M, 16, 8, 1>(L, a);
+    if (DIM == 3) launch_tiles<M, 8, 4, 4>(L, a);
+    else if (L.tile == 2) launch_tiles<M, 8, 8, 1>(L, a);
+    else if (L.tile == 3) launch_tiles<M, 32, 8, 1>(L, a);
+    else launch_tiles<M, 16, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((circle_step_c<DIM, false>), nb, L.block, 0, L.stream, a);
   }
@@ -300,10 +302,10 @@ void launch_typed(const StepLaunch &L) {
   if (L.order == 1) {
     hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
   } else if (L.tile != 0 && L.grid.reach <= 4) {
-    if (DIM == 3) launch_tile_r<CirclePolicy<F, DIM>, 8, 4, 4>(L, a);
-    else if (L.tile == 2) launch_tile_r<CirclePolicy<F, DIM>, 8, 8, 1>(L, a);
-    else if (L.tile == 3) launch_tile_r<CirclePolicy<F, DIM>, 32, 8, 1>(L, a);
-    else launch_tile_r<CirclePolicy<F, DIM>, 16, 8, 1>(L, a);
+    if (DIM == 3) launch_tiles<CirclePolicy<F, DIM>, 8, 4, 4>(L, a);
+    else if (L.tile == 2) launch_tiles<CirclePolicy<F, DIM>, 8, 8, 1>(L, a);
+    else if (L.tile == 3) launch_tiles<CirclePolicy<F, DIM>, 32, 8, 1>(L, a);
+    else launch_tiles<CirclePolicy<F, DIM>, 16, 8, 1>(L, a);
   } else {
     hipLaunchKernelGGL((circle_step<F, DIM, false>), nb, L.block, 0, L.stream, a);
   }
@@ -425,7 +427,7 @@ const ProgramDesc *program_circle() {
       {{"Point", {{"pos", K_FLOAT2, true}}}},
       {{"num_timesteps", 100}, {"num_agents", 1000}, {"rho", 0.05}, {"k_rep", 0.05},
        {"k_att", 0.01}, {"r", 5.0}, {"W", 141.421}},
-      {{"move_point", 0, 0, "r", 2.0, 0x3u, launch_move_point<2>}},
+      {{"move_point", 0, 0, "r", 2.0, 0x3u, launch_move_point<2>, false, 0, true}},
       env2,
       host_init2,
       nullptr,
@@ -442,7 +444,7 @@ const ProgramDesc *program_circle3d() {
       {{"num_timesteps", 100}, {"num_agents", 1000}, {"rho", 0.05}, {"k_rep", 0.05},
        {"k_att", 0.01}, {"r", 5.0}, {"W", 27.1442}, {"num_clusters", 64},
        {"cluster_sigma", 20.0}, {"background_fraction", 0.2}},
-      {{"move_point", 0, 0, "r", 2.0, 0x7u, launch_move_point<3>}},
+      {{"move_point", 0, 0, "r", 2.0, 0x7u, launch_move_point<3>, false, 0, true}},
       env3,
       host_init3,
       nullptr,

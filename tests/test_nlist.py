@@ -1,0 +1,107 @@
+"""Neighbour lists (hip.nlist=1, the default for steps whose near loop runs
+over their own type): per-agent lists of the halo records within radius +
+skin, built right after a (re-)sort and reused while no agent has moved more
+than skin / 2; rebuilds are decided on the device.
+
+  * with a vanishing skin every timestep rebuilds, so every step walks lists
+    built on that step's sorted order: runs are bit-identical to the stencil
+    walk (hip.nlist=0);
+  * lists too short to hold an agent's candidates (hip.list_cap) send it
+    back to the stencil walk on a fresh grid: again bit-identical;
+  * with the default skin the lists are reused for many steps: the same
+    neighbours are found (exact near test on every list entry), visited in
+    the order of the last build, so results match the stencil walk to the
+    summation-order level (over 20 boids steps in a 550-wide box: 1e-9
+    absolute in FP64, 1e-6 with the float velocities of f32c) and the FP64
+    oracle within the per-step bound of test_gpu_parity.
+"""
+import numpy as np
+import pytest
+
+from orclib import Oracle, boids2d_params, circle3d_params, circle_params, field_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+def run(model, prec, params, steps, **cfg):
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    sim = Simulation(model, prec, params=params, config=cfg)
+    sim.init_agents()
+    out = []
+    done = 0
+    for upto in steps:
+        sim.simulate(upto - done)
+        done = upto
+        out.append(sim.download(record_dtype(MEMBERS[model], prec)))
+    sim.close()
+    return out
+
+
+CASES = [
+    ("boids2d", "f64", boids2d_params(num_agents=20000), {"hip.cell_scale": 2}),
+    ("boids2d", "f32", boids2d_params(num_agents=20000), {"hip.cell_scale": 1}),
+    ("boids2d", "f32c", boids2d_params(num_agents=20000), {"hip.cell_scale": 2, "hip.tile": 3}),
+    ("circle", "f64", circle_params(num_agents=4096), {}),
+    ("circle3d", "f64", circle3d_params(num_agents=32768, rho=32768 / 1e9, r=4.0,
+                                        num_clusters=2, cluster_sigma=20.0), {}),
+    ("circle3d", "f32c", circle3d_params(num_agents=8000), {"hip.cell_scale": 2}),
+]
+IDS = [f"{c[0]}-{c[1]}-{i}" for i, c in enumerate(CASES)]
+
+
+@pytest.mark.parametrize("model,prec,params,cfg", CASES, ids=IDS)
+def test_rebuild_every_step_equals_stencil(model, prec, params, cfg):
+    print(f"case nlist tiny skin {model} {prec} {cfg}", flush=True)
+    steps = (1, 2, 5, 12)
+    a = run(model, prec, params, steps, **dict(cfg, **{"hip.skin": 1e-9}))
+    b = run(model, prec, params, steps, **dict(cfg, **{"hip.nlist": 0}))
+    for t, x, y in zip(steps, a, b):
+        assert field_bytes(x) == field_bytes(y), t
+
+
+@pytest.mark.parametrize("model,prec,params,cfg", CASES[:4], ids=IDS[:4])
+def test_overflowing_lists_equal_stencil(model, prec, params, cfg):
+    print(f"case nlist overflow {model} {prec} {cfg}", flush=True)
+    steps = (1, 3, 6)
+    a = run(model, prec, params, steps, **dict(cfg, **{"hip.list_cap": 4}))
+    b = run(model, prec, params, steps, **dict(cfg, **{"hip.nlist": 0}))
+    for t, x, y in zip(steps, a, b):
+        assert field_bytes(x) == field_bytes(y), t
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32c"])
+def test_reused_lists_match_stencil(prec):
+    params = boids2d_params(num_agents=20000)
+    steps = (1, 5, 20)
+    a = run("boids2d", prec, params, steps, **{"hip.cell_scale": 2})
+    b = run("boids2d", prec, params, steps, **{"hip.cell_scale": 2, "hip.nlist": 0})
+    # the first step walks lists built on the same sorted order: identical
+    assert field_bytes(a[0]) == field_bytes(b[0])
+    for t, x, y in zip(steps, a, b):
+        d = max(np.abs(x["pos"] - y["pos"]).max(), np.abs(x["velocity"] - y["velocity"]).max())
+        print(f"boids2d {prec} lists vs stencil after {t} steps: {d:.3e}")
+        # FP64: summation order only; f32c: float velocities (ulp 6e-8 at |v| ~ 1)
+        assert d <= (1e-9 if prec == "f64" else 1e-6)
+
+
+def test_lists_one_step_vs_oracle_after_reuse():
+    """After 30 steps on reused lists, one more step from that state agrees
+    with the FP64 oracle within 1e-12 relative."""
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    params = boids2d_params(num_agents=8192)
+    dt = record_dtype(MEMBERS["boids2d"], "f64")
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    sim = Simulation("boids2d", "f64", params=params, config={"hip.cell_scale": 2})
+    sim.init_agents()
+    sim.simulate(30)
+    s0 = sim.download(dt)
+    sim.simulate(1)
+    got = sim.download(dt)
+    o = Oracle("boids2d").set(**params)
+    o.set_agents(s0)
+    o.simulate(1)
+    want = o.agents()
+    for f in ("pos", "velocity"):
+        tol = 1e-12 * np.maximum(np.abs(want[f]), 1.0)
+        assert (np.abs(got[f] - want[f]) <= tol).all(), f
