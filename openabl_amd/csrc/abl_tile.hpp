@@ -290,15 +290,12 @@ __device__ __forceinline__ void tile_load(const typename M::Args &a, const Ctx &
   else M::load(a, j, r);
 }
 
-// Own agent k of the tile: global slot, record, FP64 position, cell and
-// fractional in-cell position.
+// Own agent k of the tile: global slot, record and FP64 position.
 template <class M, class Ctx>
 __device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ctx &c, int k,
-                                             typename M::F *own, double *pd, int3 *cell,
-                                             float *u) {
+                                             typename M::F *own, double *pd) {
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  const GridView &g = a.g;
   int orow;
   uint32_t ol;
   c.T->own(k, &orow, &ol);
@@ -317,11 +314,18 @@ __device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ct
 #pragma unroll
     for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
   }
-  *cell = cell_of<double>(g, pd[0], pd[1], pd[2]);
-  u[0] = (float) ((pd[0] - g.lo[0]) * g.inv_cell - (double) cell->x);
-  u[1] = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) cell->y);
-  u[2] = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) cell->z) : 0.f;
   return gi;
+}
+
+// The agent's cell (from its stored position, exactly as binned) and its
+// fractional position inside the cell.
+template <int DIM>
+__device__ __forceinline__ int3 own_cell(const GridView &g, const double *pd, float *u) {
+  const int3 cell = cell_of<double>(g, pd[0], pd[1], pd[2]);
+  u[0] = (float) ((pd[0] - g.lo[0]) * g.inv_cell - (double) cell.x);
+  u[1] = (float) ((pd[1] - g.lo[1]) * g.inv_cell - (double) cell.y);
+  u[2] = DIM == 3 ? (float) ((pd[2] - g.lo[2]) * g.inv_cell - (double) cell.z) : 0.f;
+  return cell;
 }
 
 // Walk the (narrowed) stencil of radius rc cell units around an own agent:
@@ -400,24 +404,42 @@ __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &
                                           const NListArgs &nl, uint32_t gi, uint32_t len, Fn &fn) {
   using F = typename M::F;
   constexpr int NS = M::NS;
-  const uint4 *lp = reinterpret_cast<const uint4 *>(nl.idx + (size_t) gi * nl.cap);
-  uint4 cur = len > 0 ? lp[0] : make_uint4(0, 0, 0, 0);
-  for (uint32_t e = 0; e < len; e += 8) {
-    const uint4 nxt = e + 8 < len ? lp[(e >> 3) + 1] : cur;
-    const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const uint32_t l = (w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
-      if constexpr (!OVF) {
-        // lists are padded to whole rows of eight with the far-away record
-        fn(c.rec + (size_t) NS * l);
-      } else if (l < (uint32_t) c.T->total) {
-        F nb[NS];
-        tile_load<M>(a, c, c.T->global_of(c.T->row_of_record(l), l), nb);
-        fn(nb);
-      }
+  const uint32_t *lp = reinterpret_cast<const uint32_t *>(nl.idx + (size_t) gi * nl.cap);
+  // four words (eight entries) per row, held as scalars: the next row is
+  // loaded while this one is visited
+  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  if (len > 0) {
+    c0 = lp[0];
+    c1 = lp[1];
+    c2 = lp[2];
+    c3 = lp[3];
+  }
+  auto one = [&](uint32_t l) {
+    if constexpr (!OVF) {
+      // lists are padded to whole rows of eight with the far-away record
+      fn(c.rec + (size_t) NS * l);
+    } else if (l < (uint32_t) c.T->total) {
+      F nb[NS];
+      tile_load<M>(a, c, c.T->global_of(c.T->row_of_record(l), l), nb);
+      fn(nb);
     }
-    cur = nxt;
+  };
+  for (uint32_t e = 0; e < len; e += 8) {
+    const bool more = e + 8 < len;
+    const uint32_t *np = lp + (more ? (e >> 1) + 4 : (e >> 1));
+    const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+    one(c0 & 0xFFFFu);
+    one(c0 >> 16);
+    one(c1 & 0xFFFFu);
+    one(c1 >> 16);
+    one(c2 & 0xFFFFu);
+    one(c2 >> 16);
+    one(c3 & 0xFFFFu);
+    one(c3 >> 16);
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
   }
 }
 
@@ -430,9 +452,9 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
-    int3 cell;
     float u[3];
-    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd);
+    const int3 cell = own_cell<M::DIM>(a.g, pd, u);
     typename M::Acc acc;
     if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
     else M::begin(a, acc, own);
@@ -475,9 +497,9 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
   for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
-    int3 cell;
     float u[3];
-    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd);
+    const int3 cell = own_cell<M::DIM>(a.g, pd, u);
     uint32_t cnt = 0;
     tile_walk<M>(a, c, cell, u, nl.rc_list, [&](const F *nb, uint32_t l) {
       double d2 = 0.0;
@@ -532,9 +554,7 @@ __global__ __launch_bounds__(256) void tiled_step_list(typename M::Args a, NList
   for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
-    int3 cell;
-    float u[3];
-    const uint32_t gi = tile_own<M>(a, c, k, own, pd, &cell, u);
+    const uint32_t gi = tile_own<M>(a, c, k, own, pd);
     typename M::Acc acc;
     if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
     else M::begin(a, acc, own);
@@ -545,6 +565,8 @@ __global__ __launch_bounds__(256) void tiled_step_list(typename M::Args a, NList
       else list_walk<M, true>(a, c, nl, gi, len, fn);
     } else {
       stray = true;
+      float u[3];
+      const int3 cell = own_cell<M::DIM>(a.g, pd, u);
       tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
     }
     if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd);
