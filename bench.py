@@ -5,8 +5,11 @@ with agent_density 0.065536 (a 4000 x 4000 box), positions uniform, velocities
 uniform in [-1,1]^2, interaction radius 10, separation radius 2, weights
 separation/alignment/cohesion 1.5/1.0/1.0, max speed 2; initial state from the
 model's own main() and the reference's fixed xorshift128+ state.  A "step" is
-one timestep of simulate(): the grid rebuild (bin, scan, scatter, stable
-rank+gather) plus the update_boid neighbour kernel, agents resident in HBM.
+one timestep of simulate(), agents resident in HBM: the update_boid neighbour
+kernel walking per-agent neighbour lists, plus the device-gated rebuild
+kernels (delta re-sort of the cell-sorted order and list build), which do
+their work on the timesteps after some agent moved more than half the list
+skin and return at once otherwise.
 
 The JSON line carries
   value            agent-steps/s, FP64, whole timestep (grid + neighbour step);
@@ -42,8 +45,10 @@ PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.
 
 # launch configuration of the runs (hip.cell_scale: cells per granularity;
 # hip.tile: LDS tile shape of the neighbour kernel; hip.block: threads per
-# block): the fastest of a sweep over tile shapes 0-3, cell scales 1-3 and
-# blocks of 128/256 on this workload (openabl_amd/autotune.py runs the same
+# block; hip.lds_slack: LDS records per expected halo agent; hip.nlist /
+# hip.skin: neighbour lists and their skin as a fraction of the radius): the
+# fastest of sweeps over tile shapes, cell scales, blocks, LDS budgets and
+# skins on this workload (openabl_amd/autotune.py runs the same kind of
 # search for any program)
 CONFIG = {"hip.cell_scale": 2, "hip.tile": 1, "hip.block": 256, "hip.lds_slack": 1.25,
           "hip.nlist": 1, "hip.skin": 0.05}
@@ -66,10 +71,9 @@ def kernel_name(precision: str) -> str:
     pol = {"f64": "BoidPolicy<double>", "f32": "BoidPolicy<float>", "f32c": "BoidPolicyC"}[precision]
     tx = {1: 16, 2: 8, 3: 32}[tile]
     if CONFIG.get("hip.nlist", 1):
-        # stencil reach of the list radius (1 + skin) * r, in cells of r / cs
-        reach = math.ceil(cs * (1.0 + CONFIG.get("hip.skin", 0.05)) / (1.0 + 2.0 ** -20)
-                          * (1.0 + 2.0 ** -21))
-        rm = 1 if reach <= 1 else (2 if reach == 2 else 4)
+        # cells are widened by 1 + skin in list mode, so the list radius
+        # (1 + skin) * r reaches cs cells
+        rm = 1 if cs <= 1 else (2 if cs == 2 else 4)
         return f"tiled_step_list<{pol}, {tx}, 8, 1, {rm}>"
     rm = 1 if cs <= 1 else (2 if cs == 2 else 4)
     return f"tiled_step<{pol}, {tx}, 8, 1, {rm}>"
@@ -91,9 +95,10 @@ def measured_traffic(kernel: str):
 
 def grid_cells(box: float, granularity: float, cell_scale: int) -> int:
     """Cells of the 2-D grid the runtime builds (cell edge granularity /
-    cell_scale, widened by 2^-20)."""
-    import math
+    cell_scale, widened by 2^-20, and by 1 + skin in neighbour-list mode)."""
     cell = granularity / cell_scale * (1.0 + 1.0 / 1048576.0)
+    if CONFIG.get("hip.nlist", 1):
+        cell *= 1.0 + CONFIG.get("hip.skin", 0.05)
     return max(1, math.ceil(box / cell)) ** 2
 
 

@@ -162,6 +162,7 @@ struct abl_sim {
   bool nlist = true;       // "hip.nlist": neighbour lists for steps that support them
   double skin = 0.05;      // "hip.skin": list skin as a fraction of the near radius
   int list_cap = 0;        // "hip.list_cap": entries per list (0: sized from the data)
+  double cell_inflate = 0.0;  // "hip.cell_inflate": cell widening factor (0: automatic)
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -511,11 +512,26 @@ __global__ void k_probe(const P *__restrict__ ax, const P *__restrict__ ay, cons
 static void *col_cur(const AgentStore &s, int c) { return s.mem[s.cur[c]][c]; }
 static void *col_other(const AgentStore &s, int c) { return s.mem[1 - s.cur[c]][c]; }
 
+// Cells are widened by (1 + skin) for a type whose near loops walk neighbour
+// lists, so the list radius (1 + skin) * r needs no more stencil reach than r
+// does ("hip.cell_inflate" overrides the factor; 1 = none).
+static double cell_inflation(const abl_sim *sim, const AgentStore &s) {
+  if (sim->cell_inflate > 0.0) return sim->cell_inflate;
+  if (!sim->nlist || sim->order != 0 || sim->tile == 0) return 1.0;
+  for (const StepDesc &st : sim->prog->steps) {
+    if (st.nlist && st.near_agent >= 0 && &sim->types[st.near_agent] == &s && !st.removes &&
+        !st.adds)
+      return 1.0 + sim->skin;
+  }
+  return 1.0;
+}
+
 static int setup_grid(abl_sim *sim, AgentStore &s) {
   double lo[3], hi[3], gran;
   sim->prog->env(sim->params.data(), lo, hi, &gran);
   if (!(gran > 0.0)) return fail(ABL_ERR_ARG, "environment granularity must be positive");
-  const double cell = gran / (double) sim->cell_scale * (1.0 + 1.0 / 1048576.0);
+  const double cell =
+      gran / (double) sim->cell_scale * (1.0 + 1.0 / 1048576.0) * cell_inflation(sim, s);
   size_t ncells = 1;
   GridView g{};
   g.dim = s.dim;
@@ -804,8 +820,12 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
   CHECK_SIM(sim);
   if (!key || !value) return fail(ABL_ERR_ARG, "null argument");
   std::string k(key), v(value);
-  // launch settings decide the lists' tiles, radius and reach: rebuild them
-  for (AgentStore &s : sim->types) s.nl_ok = false;
+  // launch settings decide the grid's cells and the lists' tiles, radius and
+  // reach: rebuild both
+  for (AgentStore &s : sim->types) {
+    s.nl_ok = false;
+    s.grid_valid = false;
+  }
   if (k == "hip.block") {
     int b = std::atoi(value);
     if (b < 64 || b > 1024 || b % 64) return fail(ABL_ERR_ARG, "hip.block must be 64..1024, a multiple of 64");
@@ -829,6 +849,11 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     if (v == "1" || v == "true") sim->nlist = true;
     else if (v == "0" || v == "false") sim->nlist = false;
     else return fail(ABL_ERR_ARG, "hip.nlist must be 0 or 1");
+  } else if (k == "hip.cell_inflate") {
+    const double f = std::atof(value);
+    if (!(f == 0.0 || (f >= 1.0 && f <= 2.0))) return fail(ABL_ERR_ARG, "hip.cell_inflate must be 0 or 1..2");
+    sim->cell_inflate = f;
+    for (AgentStore &s : sim->types) s.grid_valid = false;
   } else if (k == "hip.list_cap") {
     const int c = std::atoi(value);
     if (c < 0 || c > 65534) return fail(ABL_ERR_ARG, "hip.list_cap must be 0..65534");

@@ -54,7 +54,8 @@ IDS = [f"{c[0]}-{c[1]}-{i}" for i, c in enumerate(CASES)]
 def test_rebuild_every_step_equals_stencil(model, prec, params, cfg):
     print(f"case nlist tiny skin {model} {prec} {cfg}", flush=True)
     steps = (1, 2, 5, 12)
-    a = run(model, prec, params, steps, **dict(cfg, **{"hip.skin": 1e-9}))
+    # same cells for both (list mode would widen them by 1 + skin)
+    a = run(model, prec, params, steps, **dict(cfg, **{"hip.skin": 1e-9, "hip.cell_inflate": 1}))
     b = run(model, prec, params, steps, **dict(cfg, **{"hip.nlist": 0}))
     for t, x, y in zip(steps, a, b):
         assert field_bytes(x) == field_bytes(y), t
@@ -64,7 +65,7 @@ def test_rebuild_every_step_equals_stencil(model, prec, params, cfg):
 def test_overflowing_lists_equal_stencil(model, prec, params, cfg):
     print(f"case nlist overflow {model} {prec} {cfg}", flush=True)
     steps = (1, 3, 6)
-    a = run(model, prec, params, steps, **dict(cfg, **{"hip.list_cap": 4}))
+    a = run(model, prec, params, steps, **dict(cfg, **{"hip.list_cap": 4, "hip.cell_inflate": 1}))
     b = run(model, prec, params, steps, **dict(cfg, **{"hip.nlist": 0}))
     for t, x, y in zip(steps, a, b):
         assert field_bytes(x) == field_bytes(y), t
@@ -74,7 +75,7 @@ def test_overflowing_lists_equal_stencil(model, prec, params, cfg):
 def test_reused_lists_match_stencil(prec):
     params = boids2d_params(num_agents=20000)
     steps = (1, 5, 20)
-    a = run("boids2d", prec, params, steps, **{"hip.cell_scale": 2})
+    a = run("boids2d", prec, params, steps, **{"hip.cell_scale": 2, "hip.cell_inflate": 1})
     b = run("boids2d", prec, params, steps, **{"hip.cell_scale": 2, "hip.nlist": 0})
     # the first step walks lists built on the same sorted order: identical
     assert field_bytes(a[0]) == field_bytes(b[0])

@@ -6,7 +6,10 @@ neighbour offsets with Kahan-compensated sums.  Bars stated here, per model,
 one step from a shared FP64 state:
 
   * circle, circle3d (incl. dense clusters of ~200 neighbours):
-    |f32c - f64| <= 1e-5 absolute on positions (boxes 43-320 wide) for every
+    |f32c - f64| <= 1e-5 absolute plus 1e-6 of the agent's own FP64
+    displacement in that step (float sums of up to ~200 forces that move an
+    agent up to ~50 units in the first steps of a cluster) on positions
+    (boxes 43-320 wide), for every
     agent with no pair within 1e-4 of the distance r, where the force jumps
     from repulsion (k_rep * r) to attraction (k_att * r); such pairs are rare
     and may flip, as they do under the C backend's use_float;
@@ -91,6 +94,8 @@ def test_circle_one_step_error(model, params, tile):
     want = o.agents()
     got = f32c_step(model, params, s0, **{"hip.tile": tile})
     d = np.abs(got["pos"] - want["pos"]).max(axis=1)
+    moved = np.abs(want["pos"] - s0["pos"]).max(axis=1)
+    bar = 1e-5 + 1e-6 * moved
     flag = boundary_agents(s0["pos"], [params["r"]], None)
     # the C backend's use_float from the same state, for comparison
     o32 = Oracle(model, "f32").set(**params)
@@ -99,10 +104,9 @@ def test_circle_one_step_error(model, params, tile):
     d32 = np.abs(o32.agents()["pos"].astype(np.float64) - want["pos"]).max(axis=1)
     n = len(d)
     print(f"{model}: max |f32c - f64| after one step = {d[~flag].max():.3e} off the r boundary "
-          f"({int(flag.sum())} boundary agents, {int((d > 1e-5).sum())} beyond 1e-5); "
-          f"use_float: {d32[~flag].max():.3e}")
-    assert d[~flag].max() <= 1e-5
-    assert not ((d > 1e-5) & ~flag).any()
+          f"({int(flag.sum())} boundary agents, {int((d > bar).sum())} beyond the bar, largest "
+          f"step {moved.max():.3g}); use_float: {d32[~flag].max():.3e}")
+    assert not ((d > bar) & ~flag).any()
     assert flag.sum() <= max(8, n // 100)
     assert d[~flag].max() <= d32[~flag].max()
 
