@@ -102,12 +102,15 @@ __device__ __forceinline__ void gather_cols(const GatherCols &g, uint32_t src, u
 constexpr uint16_t kListOverflow = 0xFFFF;
 
 struct NListArgs {
-  uint16_t *idx;        // [n][cap] halo indices (cap a multiple of 8)
+  // halo indices in rows of eight entries (16 bytes): row r of agent i is
+  // idx[(r * stride + i) * 8 ...], so a wave's lanes read one row coalesced
+  uint16_t *idx;        // [cap / 8][stride][8]
   uint16_t *len;        // [n] entries, or kListOverflow
   double *pos0;         // [dim][n] positions at the build
   uint32_t *gate;       // [2] rebuild flags by timestep parity
   uint32_t *stats;      // [0]: longest list at the last build
   int n, cap, parity;
+  int stride;           // agents per row block of idx (>= n)
   float rc_list;        // (radius + skin) in cell units, widened
   double r2_list;       // (radius + skin)^2, widened
   double half_skin2;    // (skin / 2)^2

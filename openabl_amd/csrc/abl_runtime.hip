@@ -1190,6 +1190,7 @@ static NListArgs list_args(const abl_sim *sim, const AgentStore &s, double r, do
   nl.stats = s.nl_gate + 2;
   nl.n = (int) s.n;
   nl.cap = s.nl_cap;
+  nl.stride = (int) s.nl_ncap;
   nl.parity = (int) (sim->timestep & 1u);
   nl.rc_list = (float) (rl / s.cell * (1.0 + 1.0 / 1048576.0));
   // widened far past the float error of the records (relative records: ~1e-6)

@@ -48,8 +48,12 @@ struct TileTables {
 
   // Block-cooperative set-up for tile `tile` (row-major over the tile grid).
   // `cap` is the LDS record capacity; overflow is set when the halo exceeds it.
+  // One round of global reads: every thread derives the tile geometry itself,
+  // the cell offsets of all halo cells are read at once, and the first wave
+  // forms the per-row prefixes from them (blockDim.x >= 64).
   __device__ void setup(const GridView &g, int reach, unsigned tile, int cap) {
-    if (threadIdx.x == 0) {
+    int T0[3], TN[3], H0[3], HN[3];
+    {
       const int ntx = (g.n[0] + TX - 1) / TX;
       const int nty = (g.n[1] + TY - 1) / TY;
       int b[3];
@@ -57,58 +61,89 @@ struct TileTables {
       b[1] = (int) ((tile / (unsigned) ntx) % (unsigned) nty);
       b[2] = DIM == 3 ? (int) (tile / ((unsigned) ntx * (unsigned) nty)) : 0;
       const int T[3] = {TX, TY, DIM == 3 ? TZ : 1};
+#pragma unroll
       for (int d = 0; d < 3; d++) {
         if (d < DIM) {
-          t0[d] = b[d] * T[d];
-          tn[d] = min(T[d], g.n[d] - t0[d]);
-          h0[d] = max(t0[d] - reach, 0);
-          hn[d] = min(t0[d] + tn[d] + reach, g.n[d]) - h0[d];
+          T0[d] = b[d] * T[d];
+          TN[d] = min(T[d], g.n[d] - T0[d]);
+          H0[d] = max(T0[d] - reach, 0);
+          HN[d] = min(T0[d] + TN[d] + reach, g.n[d]) - H0[d];
         } else {
-          t0[d] = h0[d] = 0;
-          tn[d] = hn[d] = 1;
+          T0[d] = H0[d] = 0;
+          TN[d] = HN[d] = 1;
         }
       }
-      nrows = hn[1] * hn[2];
     }
-    __syncthreads();
-    const int nx = g.n[0], ny = g.n[1];
-    // halo rows: first slot and agent count
-    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-      const int hy = r % hn[1], hz = r / hn[1];
-      const uint32_t grow = ((uint32_t) (h0[2] + hz) * (uint32_t) ny + (uint32_t) (h0[1] + hy)) *
-                            (uint32_t) nx;
-      const uint32_t a = g.start[grow + h0[0]];
-      hstart[r] = a;
-      lbase[r + 1] = g.start[grow + h0[0] + hn[0]] - a;
-    }
-    __syncthreads();
+    const int nr = HN[1] * HN[2];
+    const int w = HN[0] + 1;
     if (threadIdx.x == 0) {
-      lbase[0] = 0;
-      for (int r = 0; r < nrows; r++) lbase[r + 1] += lbase[r];
-      total = (int) lbase[nrows];
-      // one record more than the halo: the far-away pad record of the lists
-      overflow = total >= cap ? 1 : 0;
-    }
-    __syncthreads();
-    const int w = hn[0] + 1;
-    for (int e = threadIdx.x; e < nrows * w; e += blockDim.x) {
-      const int r = e / w, hx = e % w;
-      const int hy = r % hn[1], hz = r / hn[1];
-      const uint32_t grow = ((uint32_t) (h0[2] + hz) * (uint32_t) ny + (uint32_t) (h0[1] + hy)) *
-                            (uint32_t) nx;
-      lcell[r * (HX + 1) + hx] = lbase[r] + (g.start[grow + h0[0] + hx] - hstart[r]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int oy = t0[1] - h0[1], oz = t0[2] - h0[2], ox = t0[0] - h0[0];
-      const int nq = tn[1] * tn[2];
-      ownbase[0] = 0;
-      for (int q = 0; q < nq; q++) {
-        const int r = (oz + q / tn[1]) * hn[1] + (oy + q % tn[1]);
-        ownrow[q] = (uint32_t) r;
-        ownbase[q + 1] = ownbase[q] + (cell_lds(r, ox + tn[0]) - cell_lds(r, ox));
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        t0[d] = T0[d];
+        tn[d] = TN[d];
+        h0[d] = H0[d];
+        hn[d] = HN[d];
       }
-      own_total = (int) ownbase[nq];
+      nrows = nr;
+    }
+    const int nx = g.n[0], ny = g.n[1];
+    // global offsets of every halo cell (converted to LDS indices below)
+    for (int e = threadIdx.x; e < nr * w; e += blockDim.x) {
+      const int r = e / w, hx = e % w;
+      const int hy = r % HN[1], hz = r / HN[1];
+      const uint32_t grow = ((uint32_t) (H0[2] + hz) * (uint32_t) ny + (uint32_t) (H0[1] + hy)) *
+                            (uint32_t) nx;
+      lcell[r * (HX + 1) + hx] = g.start[grow + H0[0] + hx];
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned) kWave) {
+      const int lane = (int) threadIdx.x;
+      // halo rows: first slot, agent count, prefix
+      uint32_t carry = 0;
+      for (int r0 = 0; r0 < nr; r0 += kWave) {
+        const int r = r0 + lane;
+        uint32_t cnt = 0;
+        if (r < nr) {
+          const uint32_t a = lcell[r * (HX + 1)];
+          hstart[r] = a;
+          cnt = lcell[r * (HX + 1) + HN[0]] - a;
+        }
+        const uint32_t incl = wave_incl_scan(cnt);
+        if (r < nr) lbase[r + 1] = carry + incl;
+        carry += (uint32_t) __shfl((int) incl, kWave - 1, kWave);
+      }
+      if (lane == 0) {
+        lbase[0] = 0;
+        total = (int) carry;
+        // one record more than the halo: the far-away pad record of the lists
+        overflow = (int) carry >= cap ? 1 : 0;
+      }
+      // the tile's own rows: their halo row and a prefix of their agents
+      const int oy = T0[1] - H0[1], oz = T0[2] - H0[2], ox = T0[0] - H0[0];
+      const int nq = TN[1] * TN[2];
+      uint32_t ocarry = 0;
+      for (int q0 = 0; q0 < nq; q0 += kWave) {
+        const int q = q0 + lane;
+        uint32_t cnt = 0;
+        if (q < nq) {
+          const int r = (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]);
+          ownrow[q] = (uint32_t) r;
+          cnt = lcell[r * (HX + 1) + ox + TN[0]] - lcell[r * (HX + 1) + ox];
+        }
+        const uint32_t incl = wave_incl_scan(cnt);
+        if (q < nq) ownbase[q + 1] = ocarry + incl;
+        ocarry += (uint32_t) __shfl((int) incl, kWave - 1, kWave);
+      }
+      if (lane == 0) {
+        ownbase[0] = 0;
+        own_total = (int) ocarry;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * w; e += blockDim.x) {
+      const int r = e / w, hx = e % w;
+      const int k = r * (HX + 1) + hx;
+      lcell[k] = lbase[r] + (lcell[k] - hstart[r]);
     }
     __syncthreads();
   }
@@ -177,7 +212,9 @@ namespace abl {
 //   static void load(const Args &, uint32_t slot, F *rec)     -- NS values
 //   struct Acc   { void visit(const Args &, const F *own, const F *nb); }
 //   static void begin(const Args &, Acc &, const F *own)      -- before the loop
-//   static void finish(const Args &, Acc &, const F *own, uint32_t slot)
+//   static void finish(const Args &, Acc &, const F *own, uint32_t slot, double *np)
+//       -- writes the agent's outputs to slot; np (when not null) receives
+//          the new position in FP64
 // Records are the first DIM values = position, then the members the near
 // loop reads.  Candidates are visited row by row of the (narrowed) stencil,
 // each row in cell-sorted order: the same order as for_each_candidate.
@@ -188,10 +225,23 @@ namespace abl {
 //   static void load(const Args &, uint32_t slot, F *rec, const double org[3])
 //   static void position(const Args &, uint32_t slot, double p[3])  -- FP64 position
 //   static void begin(const Args &, Acc &, const F *own, const double p[3], const double org[3])
-//   static void finish(const Args &, Acc &, const F *own, uint32_t slot, const double p[3])
+//   static void finish(const Args &, Acc &, const F *own, uint32_t slot, const double p[3],
+//                      double *np)
 // begin() receives the agent's FP64 position and the origin, so the policy
 // can remove the rounding of the agent's own float record from every offset
 // (otherwise one systematic error shared by all of its neighbour terms).
+// Waves per SIMD the list step kernel is compiled for (its register budget):
+// a policy may declare `static constexpr int kListWaves`; otherwise the
+// compiler's own choice stands.
+template <class M, class = void>
+struct policy_list_waves {
+  static constexpr int value = 1;
+};
+template <class M>
+struct policy_list_waves<M, decltype((void) M::kListWaves)> {
+  static constexpr int value = M::kListWaves;
+};
+
 template <class M, class = void>
 struct policy_relative {
   static constexpr bool value = false;
@@ -245,10 +295,13 @@ struct TileCtx {
 };
 
 // Set up tile `tile` and stage its halo records (block-cooperative).
-template <class M, int TX, int TY, int TZ, int RM>
+// issue(c) runs after the set-up and before the record loads, so global
+// reads it starts are in flight together with them.
+template <class M, int TX, int TY, int TZ, int RM, class Issue>
 __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename M::Args &a,
                                                                  unsigned tile,
-                                                                 unsigned char *smem) {
+                                                                 unsigned char *smem,
+                                                                 Issue &&issue) {
   using Ctx = TileCtx<M, TX, TY, TZ, RM>;
   using F = typename M::F;
   constexpr int DIM = M::DIM;
@@ -265,11 +318,31 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
     for (int d = 0; d < DIM; d++)
       c.org[d] = g.lo[d] + ((double) c.T->t0[d] + 0.5 * (double) c.T->tn[d]) / g.inv_cell;
   }
+  issue(c);
   if (!c.ovf) {
-    for (int k = threadIdx.x; k < c.T->total; k += blockDim.x) {
-      const uint32_t j = c.T->global_of(c.T->row_of_record((uint32_t) k), (uint32_t) k);
-      if constexpr (policy_relative<M>::value) M::load(a, j, c.rec + (size_t) NS * k, c.org);
-      else M::load(a, j, c.rec + (size_t) NS * k);
+    // kBatch records per thread in flight: all their global reads are issued
+    // before the first is written to LDS
+    constexpr int kBatch = 4;
+    const int total = c.T->total;
+    for (int k0 = threadIdx.x; k0 < total; k0 += kBatch * (int) blockDim.x) {
+      F r[kBatch][NS];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const int k = k0 + u * (int) blockDim.x;
+        if (k < total) {
+          const uint32_t j = c.T->global_of(c.T->row_of_record((uint32_t) k), (uint32_t) k);
+          if constexpr (policy_relative<M>::value) M::load(a, j, r[u], c.org);
+          else M::load(a, j, r[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const int k = k0 + u * (int) blockDim.x;
+        if (k < total) {
+#pragma unroll
+          for (int v = 0; v < NS; v++) c.rec[(size_t) NS * k + v] = r[u][v];
+        }
+      }
     }
     // record `total`: a position no near test accepts (its squares overflow),
     // the entry neighbour lists are padded with
@@ -281,6 +354,13 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
   }
   __syncthreads();
   return c;
+}
+
+template <class M, int TX, int TY, int TZ, int RM>
+__device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename M::Args &a,
+                                                                 unsigned tile,
+                                                                 unsigned char *smem) {
+  return tile_stage<M, TX, TY, TZ, RM>(a, tile, smem, [](const TileCtx<M, TX, TY, TZ, RM> &) {});
 }
 
 template <class M, class Ctx>
@@ -397,49 +477,87 @@ __device__ __forceinline__ void tile_walk(const typename M::Args &a, const Ctx &
   else tile_walk_src<M, true>(a, c, cell, u, rc, fn);
 }
 
+// Entry e of agent i's list.
+__device__ __forceinline__ size_t list_entry(const NListArgs &nl, uint32_t i, uint32_t e) {
+  return ((size_t) (e >> 3) * (size_t) nl.stride + i) * 8u + (e & 7u);
+}
+
 // Visit the records of one agent's neighbour list: eight entries per 16-byte
-// load, the next eight loaded while these are visited.
+// row, the next row loaded while this one is visited.  The caller supplies
+// row 0 (staged with the tile).
 template <class M, bool OVF, class Ctx, class Fn>
 __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &c,
-                                          const NListArgs &nl, uint32_t gi, uint32_t len, Fn &fn) {
+                                          const NListArgs &nl, uint32_t gi, uint32_t len,
+                                          uint4 row0, Fn &fn) {
   using F = typename M::F;
   constexpr int NS = M::NS;
-  const uint32_t *lp = reinterpret_cast<const uint32_t *>(nl.idx + (size_t) gi * nl.cap);
-  // four words (eight entries) per row, held as scalars: the next row is
-  // loaded while this one is visited
-  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-  if (len > 0) {
-    c0 = lp[0];
-    c1 = lp[1];
-    c2 = lp[2];
-    c3 = lp[3];
-  }
-  auto one = [&](uint32_t l) {
-    if constexpr (!OVF) {
-      // lists are padded to whole rows of eight with the far-away record
-      fn(c.rec + (size_t) NS * l);
-    } else if (l < (uint32_t) c.T->total) {
-      F nb[NS];
-      tile_load<M>(a, c, c.T->global_of(c.T->row_of_record(l), l), nb);
-      fn(nb);
+  // row r of this agent: four words at lp + r * rstride
+  const uint32_t *lp = reinterpret_cast<const uint32_t *>(nl.idx) + (size_t) gi * 4u;
+  const size_t rstride = (size_t) nl.stride * 4u;
+  // four words (eight entries) per row, held as scalars
+  uint32_t c0 = row0.x, c1 = row0.y, c2 = row0.z, c3 = row0.w;
+  if constexpr (!OVF) {
+    // lists are padded to whole rows of eight with the far-away record.
+    // Two records in flight, as in tile_walk_src: the record of the next
+    // entry is read from LDS before the current one is visited, so the
+    // visit's branches never wait on an LDS read.
+    F ra[NS], rb[NS];
+    auto rd = [&](F *r, uint32_t l) {
+#pragma unroll
+      for (int v = 0; v < NS; v++) r[v] = c.rec[(size_t) NS * l + v];
+    };
+    rd(ra, c0 & 0xFFFFu);
+    for (uint32_t e = 0; e < len; e += 8) {
+      const bool more = e + 8 < len;
+      const uint32_t *np = lp + (size_t) ((more ? e + 8 : e) >> 3) * rstride;
+      const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+      rd(rb, c0 >> 16);
+      fn(ra);
+      rd(ra, c1 & 0xFFFFu);
+      fn(rb);
+      rd(rb, c1 >> 16);
+      fn(ra);
+      rd(ra, c2 & 0xFFFFu);
+      fn(rb);
+      rd(rb, c2 >> 16);
+      fn(ra);
+      rd(ra, c3 & 0xFFFFu);
+      fn(rb);
+      rd(rb, c3 >> 16);
+      fn(ra);
+      // first entry of the next row (a re-read of entry 0 after the last row)
+      rd(ra, n0 & 0xFFFFu);
+      fn(rb);
+      c0 = n0;
+      c1 = n1;
+      c2 = n2;
+      c3 = n3;
     }
-  };
-  for (uint32_t e = 0; e < len; e += 8) {
-    const bool more = e + 8 < len;
-    const uint32_t *np = lp + (more ? (e >> 1) + 4 : (e >> 1));
-    const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-    one(c0 & 0xFFFFu);
-    one(c0 >> 16);
-    one(c1 & 0xFFFFu);
-    one(c1 >> 16);
-    one(c2 & 0xFFFFu);
-    one(c2 >> 16);
-    one(c3 & 0xFFFFu);
-    one(c3 >> 16);
-    c0 = n0;
-    c1 = n1;
-    c2 = n2;
-    c3 = n3;
+  } else {
+    auto one = [&](uint32_t l) {
+      if (l < (uint32_t) c.T->total) {
+        F nb[NS];
+        tile_load<M>(a, c, c.T->global_of(c.T->row_of_record(l), l), nb);
+        fn(nb);
+      }
+    };
+    for (uint32_t e = 0; e < len; e += 8) {
+      const bool more = e + 8 < len;
+      const uint32_t *np = lp + (size_t) ((more ? e + 8 : e) >> 3) * rstride;
+      const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+      one(c0 & 0xFFFFu);
+      one(c0 >> 16);
+      one(c1 & 0xFFFFu);
+      one(c1 >> 16);
+      one(c2 & 0xFFFFu);
+      one(c2 >> 16);
+      one(c3 & 0xFFFFu);
+      one(c3 >> 16);
+      c0 = n0;
+      c1 = n1;
+      c2 = n2;
+      c3 = n3;
+    }
   }
 }
 
@@ -459,8 +577,8 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
     if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
     else M::begin(a, acc, own);
     tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
-    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd);
-    else M::finish(a, acc, own, gi);
+    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, nullptr);
+    else M::finish(a, acc, own, gi, nullptr);
   }
 }
 
@@ -471,8 +589,8 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
 // build, every pair within the radius now was within radius + skin then, so
 // walking the list (and applying the exact near test) finds exactly the
 // stencil's neighbours, in the stencil's order of the build.
-// Entries are halo indices of the build's tile (uint16), stored transposed
-// (entry k of agent i at idx[k * n + i]) so a wave reads them coalesced.
+// Entries are halo indices of the build's tile (uint16), in rows of eight
+// interleaved over the agents (list_entry) so a wave reads them coalesced.
 // The step kernel flags a rebuild (gate of the next timestep) when an agent
 // strays past skin / 2 or a list overflowed; the rebuild kernels of the next
 // timestep run only when their gate is set.
@@ -509,7 +627,7 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
         d2 += e * e;
       }
       if (d2 <= nl.r2_list) {
-        if (cnt < (uint32_t) nl.cap) nl.idx[(size_t) gi * nl.cap + cnt] = (uint16_t) l;
+        if (cnt < (uint32_t) nl.cap) nl.idx[list_entry(nl, gi, cnt)] = (uint16_t) l;
         cnt++;
       }
     });
@@ -518,7 +636,7 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
     if (fits) {
       // pad to a whole row of eight with the far-away record (index total)
       for (uint32_t e = cnt; e < ((cnt + 7u) & ~7u); e++)
-        nl.idx[(size_t) gi * nl.cap + e] = (uint16_t) c.T->total;
+        nl.idx[list_entry(nl, gi, e)] = (uint16_t) c.T->total;
     }
     longest = max(longest, cnt);
 #pragma unroll
@@ -541,42 +659,100 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
 // One timestep of the step through the lists.  An agent whose list
 // overflowed walks the stencil (the grid is fresh: an overflow always
 // forces a rebuild on the next timestep).
+//
+// The tile's agents are taken blockDim at a time.  Their list lengths and
+// first list rows are read together with the tile's records (one round of
+// global reads) and kept in LDS; within a pass the lanes take the agents in
+// order of their number of list rows (a counting sort in LDS), so each wave
+// walks lists of about one length instead of every wave waiting for its
+// longest.  Which lane handles an agent never changes the agent's results.
 template <class M, int TX, int TY, int TZ, int RM>
-__global__ __launch_bounds__(256) void tiled_step_list(typename M::Args a, NListArgs nl) {
+__global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list(
+    typename M::Args a, NListArgs nl) {
   extern __shared__ __align__(16) unsigned char smem[];
   using F = typename M::F;
+  using Ctx = TileCtx<M, TX, TY, TZ, RM>;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
+  // buckets: list rows 0..15 (capped), 16 = overflowed list, 17 = no agent
+  constexpr uint32_t kBuckets = 18;
+  __shared__ uint32_t bucket[kBuckets];
+  __shared__ uint16_t order[256];
+  __shared__ uint16_t plen[256];
+  __shared__ uint4 prow[256];
   const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
-  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
+  const uint4 *rows = reinterpret_cast<const uint4 *>(nl.idx);
+  // list length and row 0 of own agent k (the pass's agent of this thread)
+  uint32_t my_len = 0;
+  uint4 my_row = make_uint4(0u, 0u, 0u, 0u);
+  auto fetch = [&](const Ctx &cc, int k) {
+    if (k < cc.T->own_total) {
+      int orow;
+      uint32_t ol;
+      cc.T->own(k, &orow, &ol);
+      const uint32_t gi = cc.T->global_of(orow, ol);
+      my_len = nl.len[gi];
+      my_row = rows[gi];
+    }
+  };
+  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
+                                               [&](const Ctx &cc) { fetch(cc, (int) threadIdx.x); });
+  const int own_total = c.T->own_total;
   bool stray = false;
-  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
+  for (int k0 = 0; k0 < own_total; k0 += (int) blockDim.x) {
+    if (k0 > 0) fetch(c, k0 + (int) threadIdx.x);
+    const bool valid = k0 + (int) threadIdx.x < own_total;
+    uint32_t b = kBuckets - 1;
+    if (valid) b = my_len == kListOverflow ? kBuckets - 2 : min((my_len + 7u) >> 3, kBuckets - 3);
+    plen[threadIdx.x] = (uint16_t) (valid ? my_len : 0u);
+    prow[threadIdx.x] = my_row;
+    if (threadIdx.x < kBuckets) bucket[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t rank = atomicAdd(&bucket[b], 1u);
+    __syncthreads();
+    if (threadIdx.x < (unsigned) kWave) {
+      const uint32_t v = threadIdx.x < kBuckets ? bucket[threadIdx.x] : 0u;
+      const uint32_t incl = wave_incl_scan(v);
+      if (threadIdx.x < kBuckets) bucket[threadIdx.x] = incl - v;
+    }
+    __syncthreads();
+    order[bucket[b] + rank] = (uint16_t) threadIdx.x;
+    __syncthreads();
+    const int j = order[threadIdx.x];
+    const int k = k0 + j;
+    const uint32_t len = plen[j];
+    const uint4 row0 = prow[j];
+    __syncthreads();  // the next pass rewrites order, plen, prow and bucket
+    if (k >= own_total) continue;
     F own[NS];
     double pd[3];
     const uint32_t gi = tile_own<M>(a, c, k, own, pd);
     typename M::Acc acc;
     if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
     else M::begin(a, acc, own);
-    const uint32_t len = nl.len[gi];
     if (len != kListOverflow) {
       auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
-      if (!c.ovf) list_walk<M, false>(a, c, nl, gi, len, fn);
-      else list_walk<M, true>(a, c, nl, gi, len, fn);
+      if (!c.ovf) list_walk<M, false>(a, c, nl, gi, len, row0, fn);
+      else list_walk<M, true>(a, c, nl, gi, len, row0, fn);
     } else {
       stray = true;
       float u[3];
       const int3 cell = own_cell<M::DIM>(a.g, pd, u);
       tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
     }
-    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd);
-    else M::finish(a, acc, own, gi);
+    // the build's position, read before the step's stores so its latency
+    // overlaps the epilogue
+    double p0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
     double np[3] = {0.0, 0.0, 0.0};
-    M::new_position(a, gi, np);
+    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, np);
+    else M::finish(a, acc, own, gi, np);
     double d2 = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      const double e = np[d] - nl.pos0[(size_t) d * nl.n + gi];
+      const double e = np[d] - p0[d];
       d2 += e * e;
     }
     // !(d2 <= bound) also catches NaN positions

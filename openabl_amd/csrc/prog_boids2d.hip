@@ -72,7 +72,7 @@ struct BoidAcc {
 // Everything after the near loop, written to slot i.
 template <typename F>
 __device__ __forceinline__ void boid_finish(const BoidArgs<F> &a, BoidAcc<F> s, F x, F y, F vx,
-                                            F vy, uint32_t i) {
+                                            F vy, uint32_t i, double *np_out = nullptr) {
   F chx = 0, chy = 0;  // velocity_change
   F sx = 0, sy = 0;    // steer_velocity
   if (s.icount > 0) {
@@ -116,6 +116,10 @@ __device__ __forceinline__ void boid_finish(const BoidArgs<F> &a, BoidAcc<F> s, 
   a.opy[i] = npy;
   a.ovx[i] = nvx;
   a.ovy[i] = nvy;
+  if (np_out) {
+    np_out[0] = (double) npx;
+    np_out[1] = (double) npy;
+  }
 }
 
 // Untiled: one thread per agent, candidates read from global memory (cell
@@ -143,6 +147,10 @@ template <typename F_>
 struct BoidPolicy {
   using F = F_;
   static constexpr int DIM = 2, NS = 4;
+  // register budget of the list step: 80 (six waves per SIMD) in FP64, 64
+  // (eight) in FP32, as many waves as the LDS tiles of a 1M-agent population
+  // leave room for; both compile without spills
+  static constexpr int kListWaves = sizeof(F_) == 8 ? 6 : 8;
   using Args = BoidArgs<F>;
   __device__ static void load(const Args &a, uint32_t j, F *r) {
     r[0] = a.px[j];
@@ -156,12 +164,8 @@ struct BoidPolicy {
     }
   };
   __device__ static void begin(const Args &, Acc &, const F *) {}
-  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t gi) {
-    boid_finish<F>(a, s, own[0], own[1], own[2], own[3], gi);
-  }
-  __device__ static void new_position(const Args &a, uint32_t gi, double *p) {
-    p[0] = (double) a.opx[gi];
-    p[1] = (double) a.opy[gi];
+  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t gi, double *np) {
+    boid_finish<F>(a, s, own[0], own[1], own[2], own[3], gi, np);
   }
 };
 
@@ -224,7 +228,8 @@ struct BoidAccC {
 };
 
 __device__ __forceinline__ void boid_finish_c(const BoidArgsC &a, const BoidAccC &s, double x,
-                                              double y, float vx, float vy, uint32_t i) {
+                                              double y, float vx, float vy, uint32_t i,
+                                              double *np_out = nullptr) {
   float chx = 0.f, chy = 0.f;
   if (s.icount > 0) {
     const float n = (float) s.icount;
@@ -253,6 +258,10 @@ __device__ __forceinline__ void boid_finish_c(const BoidArgsC &a, const BoidAccC
   a.opy[i] = npy;
   a.ovx[i] = nvx;
   a.ovy[i] = nvy;
+  if (np_out) {
+    np_out[0] = npx;
+    np_out[1] = npy;
+  }
 }
 
 // Untiled FP32-compensated form: offsets from the FP64 difference.
@@ -302,12 +311,8 @@ struct BoidPolicyC {
     s.res[1] = (float) ((p[1] - org[1]) - (double) own[1]);
   }
   __device__ static void finish(const Args &a, Acc &s, const float *own, uint32_t gi,
-                                const double *p) {
-    boid_finish_c(a, s, p[0], p[1], own[2], own[3], gi);
-  }
-  __device__ static void new_position(const Args &a, uint32_t gi, double *p) {
-    p[0] = a.opx[gi];
-    p[1] = a.opy[gi];
+                                const double *p, double *np) {
+    boid_finish_c(a, s, p[0], p[1], own[2], own[3], gi, np);
   }
 };
 

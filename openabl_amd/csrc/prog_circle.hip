@@ -73,13 +73,16 @@ __device__ __forceinline__ void circle_visit(const CircleArgs<F, DIM> &a, const 
 
 // out.pos = clamp(new_pos, float(1), float(W - 1.0))
 template <typename F, int DIM>
-__device__ __forceinline__ void circle_finish(const CircleArgs<F, DIM> &a, const F *np, uint32_t i) {
+__device__ __forceinline__ void circle_finish(const CircleArgs<F, DIM> &a, const F *np, uint32_t i,
+                                              double *np_out = nullptr) {
   const F lo = (F) 1;
   const F hi = (F) ((double) a.W - 1.0);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     const F v = np[d];
-    a.o[d][i] = v < lo ? lo : (v > hi ? hi : v);
+    const F c = v < lo ? lo : (v > hi ? hi : v);
+    a.o[d][i] = c;
+    if (np_out) np_out[d] = (double) c;
   }
 }
 
@@ -130,12 +133,8 @@ struct CirclePolicy {
 #pragma unroll
     for (int d = 0; d < DIM; d++) s.np[d] = own[d];
   }
-  __device__ static void finish(const Args &a, Acc &s, const F *, uint32_t i) {
-    circle_finish<F, DIM>(a, s.np, i);
-  }
-  __device__ static void new_position(const Args &a, uint32_t i, double *p) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) p[d] = (double) a.o[d][i];
+  __device__ static void finish(const Args &a, Acc &s, const F *, uint32_t i, double *np) {
+    circle_finish<F, DIM>(a, s.np, i, np);
   }
 };
 
@@ -181,11 +180,14 @@ struct CircleAccC {
 
 template <int DIM>
 __device__ __forceinline__ void circle_finish_c(const CircleArgsC<DIM> &a, const CircleAccC<DIM> &s,
-                                                const double *x, uint32_t i) {
+                                                const double *x, uint32_t i,
+                                                double *np_out = nullptr) {
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     const double v = x[d] + s.f[d].value();
-    a.o[d][i] = v < 1.0 ? 1.0 : (v > a.hi ? a.hi : v);
+    const double c = v < 1.0 ? 1.0 : (v > a.hi ? a.hi : v);
+    a.o[d][i] = c;
+    if (np_out) np_out[d] = c;
   }
 }
 
@@ -241,12 +243,9 @@ struct CirclePolicyC {
 #pragma unroll
     for (int d = 0; d < DIM; d++) s.res[d] = (float) ((p[d] - org[d]) - (double) own[d]);
   }
-  __device__ static void finish(const Args &a, Acc &s, const float *, uint32_t i, const double *p) {
-    circle_finish_c<DIM>(a, s, p, i);
-  }
-  __device__ static void new_position(const Args &a, uint32_t i, double *p) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) p[d] = a.o[d][i];
+  __device__ static void finish(const Args &a, Acc &s, const float *, uint32_t i, const double *p,
+                                double *np) {
+    circle_finish_c<DIM>(a, s, p, i, np);
   }
 };
 

@@ -79,7 +79,7 @@ struct LifePolicy {
     }
   };
   __device__ static void begin(const Args &, Acc &s, const F *) { s.living = 0; }
-  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t i) {
+  __device__ static void finish(const Args &a, Acc &s, const F *own, uint32_t i, double *) {
     a.out_alive[i] = life_rule(own[2] != (F) 0, s.living);
   }
 };
