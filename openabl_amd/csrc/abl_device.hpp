@@ -140,6 +140,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// Inclusive sum within each row of 16 lanes (DPP row shifts: no LDS
+// traffic).  Lanes shifted in from outside the row contribute 0.
+__device__ __forceinline__ uint32_t row16_incl_scan(uint32_t v) {
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xF, 0xF, true);  // row_shr:1
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xF, 0xF, true);  // row_shr:2
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xF, 0xF, true);  // row_shr:4
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xF, 0xF, true);  // row_shr:8
+  return v;
+}
+
 // Block-wide exclusive scan (blockDim.x a multiple of 64, at most 1024).
 // Returns the exclusive prefix; *total receives the block sum.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds_waves,

@@ -96,7 +96,39 @@ struct TileTables {
       lcell[r * (HX + 1) + hx] = g.start[grow + H0[0] + hx];
     }
     __syncthreads();
-    if (threadIdx.x < (unsigned) kWave) {
+    const int oy = T0[1] - H0[1], oz = T0[2] - H0[2], ox = T0[0] - H0[0];
+    const int nq = TN[1] * TN[2];
+    if (nr <= 16 && nq <= 16) {
+      // one wave: lanes 0-15 the halo rows (first slot, agent count), lanes
+      // 16-31 the tile's own rows, each group one 16-lane prefix
+      if (threadIdx.x < (unsigned) kWave) {
+        const int lane = (int) threadIdx.x;
+        uint32_t cnt = 0;
+        if (lane < nr) {
+          const uint32_t a = lcell[lane * (HX + 1)];
+          hstart[lane] = a;
+          cnt = lcell[lane * (HX + 1) + HN[0]] - a;
+        } else if (lane >= 16 && lane < 16 + nq) {
+          const int q = lane - 16;
+          const int r = (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]);
+          ownrow[q] = (uint32_t) r;
+          cnt = lcell[r * (HX + 1) + ox + TN[0]] - lcell[r * (HX + 1) + ox];
+        }
+        const uint32_t incl = row16_incl_scan(cnt);
+        if (lane < nr) lbase[lane + 1] = incl;
+        if (lane >= 16 && lane < 16 + nq) ownbase[lane - 16 + 1] = incl;
+        const uint32_t tot = (uint32_t) __builtin_amdgcn_readlane((int) incl, 15);
+        const uint32_t otot = (uint32_t) __builtin_amdgcn_readlane((int) incl, 31);
+        if (lane == 0) {
+          lbase[0] = 0;
+          total = (int) tot;
+          // one record more than the halo: the far-away pad record of the lists
+          overflow = (int) tot >= cap ? 1 : 0;
+          ownbase[0] = 0;
+          own_total = (int) otot;
+        }
+      }
+    } else if (threadIdx.x < (unsigned) kWave) {
       const int lane = (int) threadIdx.x;
       // halo rows: first slot, agent count, prefix
       uint32_t carry = 0;
@@ -119,8 +151,6 @@ struct TileTables {
         overflow = (int) carry >= cap ? 1 : 0;
       }
       // the tile's own rows: their halo row and a prefix of their agents
-      const int oy = T0[1] - H0[1], oz = T0[2] - H0[2], ox = T0[0] - H0[0];
-      const int nq = TN[1] * TN[2];
       uint32_t ocarry = 0;
       for (int q0 = 0; q0 < nq; q0 += kWave) {
         const int q = q0 + lane;
@@ -324,23 +354,54 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
     // before the first is written to LDS
     constexpr int kBatch = 4;
     const int total = c.T->total;
-    for (int k0 = threadIdx.x; k0 < total; k0 += kBatch * (int) blockDim.x) {
-      F r[kBatch][NS];
+    constexpr int kGroup = 16;
+    const int ngroups = (int) blockDim.x / kGroup;
+    if (c.T->nrows <= ngroups) {
+      // a group of 16 lanes per halo row: the row is one contiguous range of
+      // slots and of records, so no search for a record's row
+      const int r = (int) threadIdx.x / kGroup;
+      if (r < c.T->nrows) {
+        const uint32_t lb = c.T->lbase[r], le = c.T->lbase[r + 1], hs = c.T->hstart[r];
+        for (uint32_t k0 = lb + threadIdx.x % kGroup; k0 < le; k0 += kBatch * kGroup) {
+          F rr[kBatch][NS];
 #pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        const int k = k0 + u * (int) blockDim.x;
-        if (k < total) {
-          const uint32_t j = c.T->global_of(c.T->row_of_record((uint32_t) k), (uint32_t) k);
-          if constexpr (policy_relative<M>::value) M::load(a, j, r[u], c.org);
-          else M::load(a, j, r[u]);
+          for (int u = 0; u < kBatch; u++) {
+            const uint32_t k = k0 + (uint32_t) (u * kGroup);
+            if (k < le) {
+              if constexpr (policy_relative<M>::value) M::load(a, hs + (k - lb), rr[u], c.org);
+              else M::load(a, hs + (k - lb), rr[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kBatch; u++) {
+            const uint32_t k = k0 + (uint32_t) (u * kGroup);
+            if (k < le) {
+#pragma unroll
+              for (int v = 0; v < NS; v++) c.rec[(size_t) NS * k + v] = rr[u][v];
+            }
+          }
         }
       }
+    } else {
+      // more rows than groups: threads stride over the records
+      for (int k0 = threadIdx.x; k0 < total; k0 += kBatch * (int) blockDim.x) {
+        F r[kBatch][NS];
 #pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        const int k = k0 + u * (int) blockDim.x;
-        if (k < total) {
+        for (int u = 0; u < kBatch; u++) {
+          const int k = k0 + u * (int) blockDim.x;
+          if (k < total) {
+            const uint32_t j = c.T->global_of(c.T->row_of_record((uint32_t) k), (uint32_t) k);
+            if constexpr (policy_relative<M>::value) M::load(a, j, r[u], c.org);
+            else M::load(a, j, r[u]);
+          }
+        }
 #pragma unroll
-          for (int v = 0; v < NS; v++) c.rec[(size_t) NS * k + v] = r[u][v];
+        for (int u = 0; u < kBatch; u++) {
+          const int k = k0 + u * (int) blockDim.x;
+          if (k < total) {
+#pragma unroll
+            for (int v = 0; v < NS; v++) c.rec[(size_t) NS * k + v] = r[u][v];
+          }
         }
       }
     }
@@ -674,8 +735,8 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   using Ctx = TileCtx<M, TX, TY, TZ, RM>;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  // buckets: list rows 0..15 (capped), 16 = overflowed list, 17 = no agent
-  constexpr uint32_t kBuckets = 18;
+  // buckets: list rows 0..13 (capped), 14 = overflowed list, 15 = no agent
+  constexpr uint32_t kBuckets = 16;
   __shared__ uint32_t bucket[kBuckets];
   __shared__ uint16_t order[256];
   __shared__ uint16_t plen[256];
@@ -713,7 +774,7 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     __syncthreads();
     if (threadIdx.x < (unsigned) kWave) {
       const uint32_t v = threadIdx.x < kBuckets ? bucket[threadIdx.x] : 0u;
-      const uint32_t incl = wave_incl_scan(v);
+      const uint32_t incl = row16_incl_scan(v);
       if (threadIdx.x < kBuckets) bucket[threadIdx.x] = incl - v;
     }
     __syncthreads();
