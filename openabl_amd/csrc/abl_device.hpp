@@ -140,6 +140,10 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// A block-uniform value read from LDS, made wave-uniform for the compiler
+// (scalar register): branches and loops on it are then scalar, not per-lane.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Inclusive sum within each row of 16 lanes (DPP row shifts: no LDS
 // traffic).  Lanes shifted in from outside the row contribute 0.
 __device__ __forceinline__ uint32_t row16_incl_scan(uint32_t v) {

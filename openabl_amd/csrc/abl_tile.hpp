@@ -18,6 +18,8 @@
 // every candidate, so narrowing never changes which agents are accepted.
 #pragma once
 
+#include <type_traits>
+
 #include "abl_device.hpp"
 #include "abl_runtime.hpp"
 
@@ -341,7 +343,7 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
   c.rec = reinterpret_cast<F *>(smem + ((sizeof(typename Ctx::TT) + 15) & ~(size_t) 15));
   const GridView &g = a.g;
   c.T->setup(g, g.reach, tile, a.cap);
-  c.ovf = c.T->overflow != 0;
+  c.ovf = uniform(c.T->overflow) != 0;
   c.org[0] = c.org[1] = c.org[2] = 0.0;
   if constexpr (policy_relative<M>::value) {
 #pragma unroll
@@ -353,10 +355,10 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
     // kBatch records per thread in flight: all their global reads are issued
     // before the first is written to LDS
     constexpr int kBatch = 4;
-    const int total = c.T->total;
+    const int total = uniform(c.T->total);
     constexpr int kGroup = 16;
     const int ngroups = (int) blockDim.x / kGroup;
-    if (c.T->nrows <= ngroups) {
+    if (uniform(c.T->nrows) <= ngroups) {
       // a group of 16 lanes per halo row: the row is one contiguous range of
       // slots and of records, so no search for a record's row
       const int r = (int) threadIdx.x / kGroup;
@@ -431,17 +433,18 @@ __device__ __forceinline__ void tile_load(const typename M::Args &a, const Ctx &
   else M::load(a, j, r);
 }
 
-// Own agent k of the tile: global slot, record and FP64 position.
-template <class M, class Ctx>
-__device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ctx &c, int k,
-                                             typename M::F *own, double *pd) {
+// Own agent k of the tile: global slot, record and FP64 position; OVF: the
+// records are in global memory (the halo did not fit LDS).
+template <class M, bool OVF, class Ctx>
+__device__ __forceinline__ uint32_t tile_own_src(const typename M::Args &a, const Ctx &c, int k,
+                                                 typename M::F *own, double *pd) {
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
   int orow;
   uint32_t ol;
   c.T->own(k, &orow, &ol);
   const uint32_t gi = c.T->global_of(orow, ol);
-  if (!c.ovf) {
+  if constexpr (!OVF) {
 #pragma unroll
     for (int v = 0; v < NS; v++) own[v] = c.rec[(size_t) NS * ol + v];
   } else {
@@ -456,6 +459,13 @@ __device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ct
     for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
   }
   return gi;
+}
+
+template <class M, class Ctx>
+__device__ __forceinline__ uint32_t tile_own(const typename M::Args &a, const Ctx &c, int k,
+                                             typename M::F *own, double *pd) {
+  if (!c.ovf) return tile_own_src<M, false>(a, c, k, own, pd);
+  return tile_own_src<M, true>(a, c, k, own, pd);
 }
 
 // The agent's cell (from its stored position, exactly as binned) and its
@@ -567,10 +577,19 @@ __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &
 #pragma unroll
       for (int v = 0; v < NS; v++) r[v] = c.rec[(size_t) NS * l + v];
     };
+    // rows are read two ahead: row k + 1 is in m0..m3 while row k is
+    // visited and row k + 2 is in flight
+    uint32_t m0, m1, m2, m3;
+    {
+      const uint32_t *np = lp + (len > 8 ? rstride : 0);
+      m0 = np[0];
+      m1 = np[1];
+      m2 = np[2];
+      m3 = np[3];
+    }
     rd(ra, c0 & 0xFFFFu);
     for (uint32_t e = 0; e < len; e += 8) {
-      const bool more = e + 8 < len;
-      const uint32_t *np = lp + (size_t) ((more ? e + 8 : e) >> 3) * rstride;
+      const uint32_t *np = lp + (size_t) ((e + 16 < len ? e + 16 : e) >> 3) * rstride;
       const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
       rd(rb, c0 >> 16);
       fn(ra);
@@ -586,13 +605,17 @@ __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &
       fn(rb);
       rd(rb, c3 >> 16);
       fn(ra);
-      // first entry of the next row (a re-read of entry 0 after the last row)
-      rd(ra, n0 & 0xFFFFu);
+      // first entry of the next row (a re-read after the last row)
+      rd(ra, m0 & 0xFFFFu);
       fn(rb);
-      c0 = n0;
-      c1 = n1;
-      c2 = n2;
-      c3 = n3;
+      c0 = m0;
+      c1 = m1;
+      c2 = m2;
+      c3 = m3;
+      m0 = n0;
+      m1 = n1;
+      m2 = n2;
+      m3 = n3;
     }
   } else {
     auto one = [&](uint32_t l) {
@@ -628,7 +651,8 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   using F = typename M::F;
   constexpr int NS = M::NS;
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
-  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
+  const int own_total = uniform(c.T->own_total);
+  for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
     float u[3];
@@ -673,7 +697,8 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
   }
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
   uint32_t longest = 0;
-  for (int k = threadIdx.x; k < c.T->own_total; k += blockDim.x) {
+  const int own_total = uniform(c.T->own_total);
+  for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
     float u[3];
@@ -759,7 +784,7 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   };
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
                                                [&](const Ctx &cc) { fetch(cc, (int) threadIdx.x); });
-  const int own_total = c.T->own_total;
+  const int own_total = uniform(c.T->own_total);
   bool stray = false;
   for (int k0 = 0; k0 < own_total; k0 += (int) blockDim.x) {
     if (k0 > 0) fetch(c, k0 + (int) threadIdx.x);
@@ -786,36 +811,43 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     const uint4 row0 = prow[j];
     __syncthreads();  // the next pass rewrites order, plen, prow and bucket
     if (k >= own_total) continue;
-    F own[NS];
-    double pd[3];
-    const uint32_t gi = tile_own<M>(a, c, k, own, pd);
-    typename M::Acc acc;
-    if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
-    else M::begin(a, acc, own);
-    if (len != kListOverflow) {
-      auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
-      if (!c.ovf) list_walk<M, false>(a, c, nl, gi, len, row0, fn);
-      else list_walk<M, true>(a, c, nl, gi, len, row0, fn);
-    } else {
-      stray = true;
-      float u[3];
-      const int3 cell = own_cell<M::DIM>(a.g, pd, u);
-      tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
-    }
-    // the build's position, read before the step's stores so its latency
-    // overlaps the epilogue
-    double p0[3] = {0.0, 0.0, 0.0};
+    // the whole agent body is specialised on where the records are, so the
+    // LDS path never waits on global reads of the overflow path
+    auto agent = [&](auto src) {
+      constexpr bool OVF = decltype(src)::value;
+      F own[NS];
+      double pd[3];
+      const uint32_t gi = tile_own_src<M, OVF>(a, c, k, own, pd);
+      typename M::Acc acc;
+      if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
+      else M::begin(a, acc, own);
+      if (len != kListOverflow) {
+        auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
+        list_walk<M, OVF>(a, c, nl, gi, len, row0, fn);
+      } else {
+        stray = true;
+        float u[3];
+        const int3 cell = own_cell<M::DIM>(a.g, pd, u);
+        auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
+        tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+      }
+      // the build's position, read before the step's stores so its latency
+      // overlaps the epilogue
+      double p0[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
-    double np[3] = {0.0, 0.0, 0.0};
-    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, np);
-    else M::finish(a, acc, own, gi, np);
-    double d2 = 0.0;
+      for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
+      double np[3] = {0.0, 0.0, 0.0};
+      if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, np);
+      else M::finish(a, acc, own, gi, np);
+      double d2 = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      const double e = np[d] - p0[d];
-      d2 += e * e;
-    }
+      for (int d = 0; d < DIM; d++) {
+        const double e = np[d] - p0[d];
+        d2 += e * e;
+      }
+      return d2;
+    };
+    const double d2 = c.ovf ? agent(std::true_type{}) : agent(std::false_type{});
     // !(d2 <= bound) also catches NaN positions
     if (!(d2 <= nl.half_skin2)) stray = true;
   }
