@@ -124,7 +124,14 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
 /* `-C key=value` runtime options: "hip.block" (threads per block),
  * "hip.cell_scale" (cells per granularity, 1..4), "hip.order" ("cell" or
  * "index": visit neighbours in cell order or in the reference's index
- * order), "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
+ * order), "hip.tile" (LDS tile shape, 0 = untiled), "hip.lds_slack" (LDS
+ * records per tile as a multiple of the expected halo), "hip.grid" ("delta"
+ * or "full" re-sort), "hip.nlist" (1: neighbour lists with a skin for tiled
+ * near loops over the agent's own type, 0: walk the stencil every step),
+ * "hip.skin" (list skin as a fraction of the radius), "hip.list_cap" (list
+ * entries per agent; a longer list falls back to the stencil walk),
+ * "hip.cell_inflate" (cell widening factor; list mode widens by 1 + skin),
+ * "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
  * the counter-based generator random() uses inside steps), "hip.log_csv"
  * (file log_csv rows are appended to). */
 int abl_hip_set_config(abl_sim *sim, const char *key, const char *value);
