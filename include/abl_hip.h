@@ -131,6 +131,11 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
  * "hip.skin" (list skin as a fraction of the radius), "hip.list_cap" (list
  * entries per agent; a longer list falls back to the stencil walk),
  * "hip.cell_inflate" (cell widening factor; list mode widens by 1 + skin),
+ * "hip.rebuild" ("fused": a list rebuild's re-sort and build run as one
+ * launch with grid barriers; "split": four launches), "hip.barrier_spins"
+ * (spin limit of those barriers; a launch that runs out abandons the rebuild,
+ * the next synchronize or download returns ABL_ERR_DEVICE and the runtime
+ * rebuilds from scratch with the split launches from then on),
  * "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
  * the counter-based generator random() uses inside steps), "hip.log_csv"
  * (file log_csv rows are appended to). */

@@ -120,18 +120,18 @@ static __global__ __launch_bounds__(kScanBlock) void k_tile_totals(const uint32_
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ count,
-                                                                  const uint32_t *__restrict__ bsum,
-                                                                  uint32_t *__restrict__ start,
-                                                                  int ncells,
-                                                                  const uint32_t *__restrict__ gate = nullptr) {
-  if (gate && *gate == 0u) return;
-  __shared__ uint32_t lds[kWave + 1];
+// Scan tile `tile` of `ntiles` (block-cooperative, kScanBlock threads; lds
+// holds kWave + 1 words).
+__device__ __forceinline__ void scan_apply_tile(int tile, int ntiles,
+                                                const uint32_t *__restrict__ count,
+                                                const uint32_t *__restrict__ bsum,
+                                                uint32_t *__restrict__ start, int ncells,
+                                                uint32_t *lds) {
   uint32_t part = 0;
-  for (int t = threadIdx.x; t < (int) blockIdx.x; t += blockDim.x) part += bsum[t];
+  for (int t = threadIdx.x; t < tile; t += blockDim.x) part += bsum[t];
   uint32_t carry;
   (void) block_excl_scan(part, lds, &carry);
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const int base = tile * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t sum = 0;
 #pragma unroll
@@ -146,7 +146,17 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t
     if (base + k < ncells) start[base + k] = pre;
     pre += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = carry + total;
+  if (tile == ntiles - 1 && threadIdx.x == 0) start[ncells] = carry + total;
+}
+
+static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ count,
+                                                                  const uint32_t *__restrict__ bsum,
+                                                                  uint32_t *__restrict__ start,
+                                                                  int ncells,
+                                                                  const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
+  __shared__ uint32_t lds[kWave + 1];
+  scan_apply_tile((int) blockIdx.x, (int) gridDim.x, count, bsum, start, ncells, lds);
 }
 
 // ---- delta re-sort -------------------------------------------------------------
@@ -169,15 +179,13 @@ struct MoverLists {
 };
 
 template <typename P>
-static __global__ void k_bin_delta(const P *__restrict__ px, const P *__restrict__ py,
-                                   const P *__restrict__ pz, GridView g,
-                                   const uint32_t *__restrict__ skey, uint32_t *__restrict__ key,
-                                   uint32_t *__restrict__ count, uint32_t *__restrict__ bsum,
-                                   MoverLists ml, uint32_t gen, int n,
-                                   const uint32_t *__restrict__ gate = nullptr) {
-  if (gate && *gate == 0u) return;  // neighbour-list mode: no rebuild this timestep
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
+                                               const P *__restrict__ py, const P *__restrict__ pz,
+                                               const GridView &g, const uint32_t *__restrict__ skey,
+                                               uint32_t *__restrict__ key,
+                                               uint32_t *__restrict__ count,
+                                               uint32_t *__restrict__ bsum, const MoverLists &ml,
+                                               uint32_t gen, int n) {
   const int3 c = cell_of<P>(g, px[i], py[i], g.dim == 3 ? pz[i] : (P) 0);
   const uint32_t k = ((uint32_t) c.z * (uint32_t) g.n[1] + (uint32_t) c.y) * (uint32_t) g.n[0] +
                      (uint32_t) c.x;
@@ -198,16 +206,28 @@ static __global__ void k_bin_delta(const P *__restrict__ px, const P *__restrict
   }
 }
 
-static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t *__restrict__ skey_in,
-                               uint32_t *__restrict__ skey_out,
-                               const uint32_t *__restrict__ start_old,
-                               const uint32_t *__restrict__ start_new, MoverLists ml, uint32_t gen,
-                               GatherCols g, const int32_t *__restrict__ id_in,
-                               int32_t *__restrict__ id_out, int n,
-                               const uint32_t *__restrict__ gate = nullptr) {
-  if (gate && *gate == 0u) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  // the mover counters start from zero at the next build (k_bin_delta is done
+template <typename P>
+static __global__ void k_bin_delta(const P *__restrict__ px, const P *__restrict__ py,
+                                   const P *__restrict__ pz, GridView g,
+                                   const uint32_t *__restrict__ skey, uint32_t *__restrict__ key,
+                                   uint32_t *__restrict__ count, uint32_t *__restrict__ bsum,
+                                   MoverLists ml, uint32_t gen, int n,
+                                   const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;  // neighbour-list mode: no rebuild this timestep
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bin_delta_item<P>(i, px, py, pz, g, skey, key, count, bsum, ml, gen, n);
+}
+
+__device__ __forceinline__ void place_item(int s, const uint32_t *__restrict__ key,
+                                           const uint32_t *__restrict__ skey_in,
+                                           uint32_t *__restrict__ skey_out,
+                                           const uint32_t *__restrict__ start_old,
+                                           const uint32_t *__restrict__ start_new,
+                                           const MoverLists &ml, uint32_t gen, const GatherCols &g,
+                                           const int32_t *__restrict__ id_in,
+                                           int32_t *__restrict__ id_out, int n) {
+  // the mover counters start from zero at the next build (binning is done
   // with this generation's counter)
   if (s == 0) {
     ml.nmov[0] = 0u;
@@ -237,6 +257,120 @@ static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t 
   skey_out[dst] = k;
   id_out[dst] = id_in[s];
   gather_cols(g, (uint32_t) s, dst);
+}
+
+// ---- the gated re-sort in one launch --------------------------------------------
+// In neighbour-list mode the re-sort runs only on timesteps whose gate the
+// previous step set.  Its kernels (bin, scan, place, then the list build)
+// can run as one launch separated by grid barriers, so a timestep without a
+// rebuild costs one launch that returns at once instead of four.
+
+// Grid-barrier state of one agent type (device memory, zero between launches).
+struct GridBarrier {
+  uint32_t count;   // arrivals of this launch
+  uint32_t exited;  // blocks done with this launch
+  uint32_t error;   // set when a barrier wait ran out: the launch was abandoned
+  uint32_t pad;
+};
+
+// Everything the re-sort phases read and write.
+struct DeltaArgs {
+  const void *px, *py, *pz;   // positions (current buffers)
+  int pos_elem;               // 8 (double) or 4 (float)
+  GridView g;                 // grid the agents are binned on
+  const uint32_t *skey;       // sorted keys of the stored order
+  uint32_t *key, *count, *bsum;
+  MoverLists ml;
+  uint32_t gen;
+  int n;
+  const uint32_t *start_old;
+  uint32_t *start_new;
+  int ncells, nscan;          // cells, scan tiles
+  uint32_t *skey_out;
+  GatherCols gc;              // current -> other buffers
+  const int32_t *id_in;
+  int32_t *id_out;
+  GridBarrier *bar;
+  uint32_t spins;             // spin limit of a barrier wait (kBarrierSpins)
+  int *launched;              // host: set to 1 when the launch function launched it
+};
+
+// Spin limit of one barrier wait (each spin sleeps ~0.25 us): far above any
+// phase of a rebuild, so it only ends a launch whose blocks are not all
+// resident.
+constexpr uint32_t kBarrierSpins = 1u << 20;
+
+// Barrier across all blocks of a launch whose blocks are all resident at once
+// (the grid is at most half of what the CUs hold).  Thread 0 of each block
+// arrives with an agent-scope atomic and waits with s_sleep.  A wait that runs
+// out sets bar->error and returns false, as does any wait once the error is
+// set: every block then leaves the launch, so no wave waits for ever.
+__device__ __forceinline__ bool grid_barrier(GridBarrier *bar, uint32_t target, uint32_t spins) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int good = 1;
+    for (uint32_t spin = 0;; spin++) {
+      if (__hip_atomic_load(&bar->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (spin >= spins ||
+          __hip_atomic_load(&bar->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        __hip_atomic_fetch_or(&bar->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+    __threadfence();
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// The last block out of a launch zeroes the arrival counters for the next one.
+__device__ __forceinline__ void grid_barrier_exit(GridBarrier *bar) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t e =
+        __hip_atomic_fetch_add(&bar->exited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (e == gridDim.x - 1u) {
+      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar->exited, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Phases 1-3 of the gated re-sort (grid-stride; blockDim.x == kScanBlock).
+// Returns false when a barrier was abandoned.
+template <typename P>
+__device__ __forceinline__ bool delta_resort(const DeltaArgs &d, uint32_t *lds) {
+  const int stride = (int) (gridDim.x * blockDim.x);
+  const int first = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+  for (int i = first; i < d.n; i += stride)
+    bin_delta_item<P>(i, (const P *) d.px, (const P *) d.py, (const P *) d.pz, d.g, d.skey, d.key,
+                      d.count, d.bsum, d.ml, d.gen, d.n);
+  if (!grid_barrier(d.bar, gridDim.x, d.spins)) return false;
+  for (int t = (int) blockIdx.x; t < d.nscan; t += (int) gridDim.x)
+    scan_apply_tile(t, d.nscan, d.count, d.bsum, d.start_new, d.ncells, lds);
+  if (!grid_barrier(d.bar, 2u * gridDim.x, d.spins)) return false;
+  for (int s = first; s < d.n; s += stride)
+    place_item(s, d.key, d.skey, d.skey_out, d.start_old, d.start_new, d.ml, d.gen, d.gc, d.id_in,
+               d.id_out, d.n);
+  return grid_barrier(d.bar, 3u * gridDim.x, d.spins);
+}
+
+static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t *__restrict__ skey_in,
+                               uint32_t *__restrict__ skey_out,
+                               const uint32_t *__restrict__ start_old,
+                               const uint32_t *__restrict__ start_new, MoverLists ml, uint32_t gen,
+                               GatherCols g, const int32_t *__restrict__ id_in,
+                               int32_t *__restrict__ id_out, int n,
+                               const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
+  place_item((int) (blockIdx.x * blockDim.x + threadIdx.x), key, skey_in, skey_out, start_old,
+             start_new, ml, gen, g, id_in, id_out, n);
 }
 
 

@@ -134,7 +134,8 @@ struct AgentStore {
   bool nl_ok = false;
   uint16_t *nl_idx = nullptr, *nl_len = nullptr;
   double *nl_pos0 = nullptr;
-  uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list
+  uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list,
+                                // [4..7] grid barrier of the one-launch rebuild
   int nl_cap = 0;               // entries per agent
   size_t nl_ncap = 0;           // agents allocated
   // run-time add/remove: ids stop being 0..n-1 once agents die or are born
@@ -163,6 +164,8 @@ struct abl_sim {
   double skin = 0.05;      // "hip.skin": list skin as a fraction of the near radius
   int list_cap = 0;        // "hip.list_cap": entries per list (0: sized from the data)
   double cell_inflate = 0.0;  // "hip.cell_inflate": cell widening factor (0: automatic)
+  bool fused_rebuild = true;  // "hip.rebuild": gated re-sort + list build in one launch
+  uint32_t barrier_spins = kBarrierSpins;  // "hip.barrier_spins": its grid barriers' spin limit
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -870,6 +873,24 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     if (v == "delta") sim->grid_delta = true;
     else if (v == "full") sim->grid_delta = false;
     else return fail(ABL_ERR_ARG, "hip.grid must be delta or full");
+  } else if (k == "hip.rebuild") {
+    if (v == "fused") {
+      // a fresh barrier state (an abandoned launch leaves its error flag set)
+      for (AgentStore &s : sim->types) {
+        if (s.nl_gate) {
+          HIP_TRY(hipStreamSynchronize(sim->stream));
+          HIP_TRY(hipMemset(s.nl_gate + 4, 0, 4 * 4));
+        }
+      }
+      sim->fused_rebuild = true;
+    } else if (v == "split") {
+      sim->fused_rebuild = false;
+    }
+    else return fail(ABL_ERR_ARG, "hip.rebuild must be fused or split");
+  } else if (k == "hip.barrier_spins") {
+    const long long sp = std::atoll(value);
+    if (sp < 0 || sp > (long long) kBarrierSpins) return fail(ABL_ERR_ARG, "hip.barrier_spins out of range");
+    sim->barrier_spins = (uint32_t) sp;
   } else if (k == "hip.seed") {
     sim->seed = std::strtoull(value, nullptr, 0);
   } else if (k == "hip.log_csv") {
@@ -941,6 +962,8 @@ extern "C" int abl_hip_count(abl_sim *sim, int t, size_t *count) {
   return ABL_OK;
 }
 
+static int check_rebuild(abl_sim *sim);
+
 extern "C" int abl_hip_download(abl_sim *sim, int t, void *aos, size_t capacity, size_t *count) {
   CHECK_SIM(sim);
   if (t < 0 || t >= (int) sim->types.size()) return fail(ABL_ERR_ARG, "agent type out of range");
@@ -948,7 +971,10 @@ extern "C" int abl_hip_download(abl_sim *sim, int t, void *aos, size_t capacity,
   if (count) *count = s.n;
   if (s.n == 0) return ABL_OK;
   if (!aos || capacity < s.n) return fail(ABL_ERR_ARG, "download buffer too small");
-  int rc = reserve_staging(sim, s.n * s.aos_size);
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  int rc = check_rebuild(sim);
+  if (rc) return rc;
+  rc = reserve_staging(sim, s.n * s.aos_size);
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(sim->staging, 0, s.n * s.aos_size, sim->stream));
   ColXfer x = xfer_of(s, true);
@@ -1175,7 +1201,10 @@ static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
   HIP_TRY(hipMalloc(&s.nl_idx, (size_t) cap * n * 2));
   HIP_TRY(hipMalloc(&s.nl_len, n * 2));
   HIP_TRY(hipMalloc(&s.nl_pos0, 3 * n * 8));
-  if (!s.nl_gate) HIP_TRY(hipMalloc(&s.nl_gate, 4 * 4));
+  if (!s.nl_gate) {
+    HIP_TRY(hipMalloc(&s.nl_gate, 8 * 4));
+    HIP_TRY(hipMemset(s.nl_gate, 0, 8 * 4));
+  }
   s.nl_cap = cap;
   s.nl_ncap = n;
   return ABL_OK;
@@ -1248,7 +1277,38 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
     placed.dst[c] = col_other(b, c);
     placed.elem[c] = b.cols[c].elem;
   }
-  launch_delta(sim, b, gate, start_cur, start_other, placed);
+  // the re-sort in one launch with the list build (tiled_rebuild), when the
+  // program's launch function can run it; else four gated launches
+  DeltaArgs d{};
+  int launched = 0;
+  if (sim->fused_rebuild && b.n > 0 && b.nmov) {
+    const int pc = b.pos_col;
+    d.px = col_cur(b, pc);
+    d.py = col_cur(b, pc + 1);
+    d.pz = b.dim == 3 ? col_cur(b, pc + 2) : nullptr;
+    d.pos_elem = b.cols[pc].elem;
+    d.g = b.gv;
+    d.skey = b.skey[b.skey_cur];
+    d.key = b.key;
+    d.count = b.count;
+    d.bsum = b.bsum;
+    d.ml = MoverLists{b.dirty, b.mhead, b.mnext, b.mslot, b.nmov};
+    d.gen = b.gen;
+    d.n = (int) b.n;
+    d.start_old = start_cur;
+    d.start_new = start_other;
+    d.ncells = (int) b.ncells;
+    d.nscan = std::max(((int) b.ncells + kScanTile - 1) / kScanTile, 1);
+    d.skey_out = b.skey[1 - b.skey_cur];
+    d.gc = placed;
+    d.id_in = b.id[b.id_cur];
+    d.id_out = b.id[1 - b.id_cur];
+    d.bar = reinterpret_cast<GridBarrier *>(b.nl_gate + 4);
+    d.spins = sim->barrier_spins;
+    d.launched = &launched;
+  } else {
+    launch_delta(sim, b, gate, start_cur, start_other, placed);
+  }
   NListArgs nl = list_args(sim, b, r, rl);
   nl.gated = 1;
   nl.copy.ncols = placed.ncols;
@@ -1266,10 +1326,42 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
   StepLaunch L = Lb;
   for (size_t c = 0; c < b.cols.size(); c++) L.in.c[c] = col_other(b, (int) c);
   L.grid.start = start_other;
-  L.nl_mode = 1;
   L.nl = &nl;
+  if (d.launched) {
+    L.nl_mode = 3;
+    L.delta = &d;
+    st.launch(L);
+    HIP_TRY(hipGetLastError());
+    if (launched) return ABL_OK;
+    launch_delta(sim, b, gate, start_cur, start_other, placed);
+  }
+  L.nl_mode = 1;
+  L.delta = nullptr;
   st.launch(L);
   HIP_TRY(hipGetLastError());
+  return ABL_OK;
+}
+
+// A one-launch rebuild that abandoned a grid barrier (its blocks were not all
+// resident) left its agent type half re-sorted: report it at the next
+// synchronisation, rebuild from scratch from then on and stop using it.
+static int check_rebuild(abl_sim *sim) {
+  for (AgentStore &s : sim->types) {
+    if (!s.nl_gate) continue;
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, s.nl_gate + 6, 4, hipMemcpyDeviceToHost));
+    if (err) {
+      HIP_TRY(hipMemset(s.nl_gate + 4, 0, 4 * 4));
+      sim->fused_rebuild = false;
+      for (AgentStore &t : sim->types) {
+        t.nl_ok = false;
+        t.grid_valid = false;
+        t.sorted = false;
+      }
+      return fail(ABL_ERR_DEVICE, "one-launch neighbour-list rebuild abandoned a grid barrier; "
+                                  "results since the last synchronisation are invalid");
+    }
+  }
   return ABL_OK;
 }
 
@@ -1441,7 +1533,7 @@ extern "C" int abl_hip_simulate(abl_sim *sim, int timesteps) {
 extern "C" int abl_hip_synchronize(abl_sim *sim) {
   CHECK_SIM(sim);
   HIP_TRY(hipStreamSynchronize(sim->stream));
-  return ABL_OK;
+  return check_rebuild(sim);
 }
 
 extern "C" int abl_hip_query(abl_sim *sim, int *done) {

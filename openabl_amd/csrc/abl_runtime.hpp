@@ -44,6 +44,7 @@ struct ParamDesc {
 
 struct StepLaunch;
 struct ColSet;
+struct DeltaArgs;
 
 // Hooks a single-type, single-step program provides so many independent runs
 // of it can be batched into one launch per kernel (abl_ensemble.hip).
@@ -120,9 +121,11 @@ struct StepLaunch {
   int tile;            // LDS tile variant (0: untiled thread-per-agent kernel)
   float occupancy;     // near-type agents per cell (sizes the LDS tile budget)
   float lds_slack;     // LDS records per expected halo agent
-  // neighbour lists: 0 = stencil walk, 1 = build the lists, 2 = step through them
+  // neighbour lists: 0 = stencil walk, 1 = build the lists, 2 = step through
+  // them, 3 = gated re-sort and build in one launch (delta, abl_pipeline.hpp)
   int nl_mode;
   const NListArgs *nl;
+  const DeltaArgs *delta;
   float rc;            // near radius in cell units, widened by 2^-20
   // run-time add()/removeCurrent() and random() inside steps
   const int32_t *self_id;  // ids of the stepping agents

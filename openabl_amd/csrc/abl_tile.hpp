@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "abl_device.hpp"
+#include "abl_pipeline.hpp"
 #include "abl_runtime.hpp"
 
 namespace abl {
@@ -203,6 +204,14 @@ struct TileTables {
     *rec = cell_lds(r, t0[0] - h0[0]) + ((uint32_t) k - ownbase[lo]);
   }
 };
+
+// Tiles of TX x TY (x TZ) cells covering the grid.
+template <int TX, int TY, int TZ, int DIM>
+__host__ __device__ inline unsigned tile_count(const GridView &g) {
+  unsigned n = (unsigned) (((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY));
+  if (DIM == 3) n *= (unsigned) ((g.n[2] + TZ - 1) / TZ);
+  return n;
+}
 
 // Conservative x-interval of cells, relative to the agent's own cell, that
 // can hold agents within `rc` cell units of an agent at fractional in-cell
@@ -681,20 +690,13 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
 // timestep run only when their gate is set.
 // Build the lists of one tile (after a sort; records come from the sorted
 // buffers a points at).
+// Build the lists of one tile's agents; returns the longest list.
 template <class M, int TX, int TY, int TZ, int RM>
-__global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NListArgs nl) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  if (nl.gated && nl.gate[nl.parity] == 0u) return;
+__device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, const NListArgs &nl,
+                                                    unsigned tile, unsigned char *smem) {
   using F = typename M::F;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
-  // the new cell offsets become the current ones (block-strided share)
-  if (nl.start_dst) {
-    for (int k = (int) (blockIdx.x * blockDim.x + threadIdx.x); k <= nl.ncells;
-         k += (int) (gridDim.x * blockDim.x))
-      nl.start_dst[k] = nl.start_src[k];
-  }
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
   uint32_t longest = 0;
   const int own_total = uniform(c.T->own_total);
@@ -733,13 +735,64 @@ __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NLis
       nl.skey_dst[gi] = nl.skey_src[gi];
     }
   }
-  // one atomic per block for the longest list
+  return longest;
+}
+
+// The new cell offsets become the current ones (grid-strided share) after a
+// gated re-sort.
+__device__ __forceinline__ void list_copy_starts(const NListArgs &nl) {
+  if (!nl.start_dst) return;
+  for (int k = (int) (blockIdx.x * blockDim.x + threadIdx.x); k <= nl.ncells;
+       k += (int) (gridDim.x * blockDim.x))
+    nl.start_dst[k] = nl.start_src[k];
+}
+
+// One atomic per block for the longest list.
+__device__ __forceinline__ void list_longest(const NListArgs &nl, uint32_t longest) {
   __shared__ uint32_t blk_longest;
   if (threadIdx.x == 0) blk_longest = 0u;
   __syncthreads();
   if (longest) atomicMax(&blk_longest, longest);
   __syncthreads();
   if (threadIdx.x == 0 && blk_longest) atomicMax(&nl.stats[0], blk_longest);
+}
+
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NListArgs nl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (nl.gated && nl.gate[nl.parity] == 0u) return;
+  list_copy_starts(nl);
+  const uint32_t longest =
+      list_build_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(blockIdx.x, gridDim.x), smem);
+  list_longest(nl, longest);
+}
+
+// Positions as stored: FP64 for FP32-compensated policies, else the policy's type.
+template <class M>
+using policy_pos_t = typename std::conditional<policy_relative<M>::value, double, typename M::F>::type;
+
+// The gated re-sort and list build of one timestep in one launch (256
+// threads, all blocks resident): bin, scan, place (abl_pipeline.hpp,
+// delta_resort), then every tile's lists.  Returns at once when the previous
+// step set no gate, or when an earlier launch abandoned a barrier (the host
+// then sees bar->error and rebuilds from scratch).
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListArgs nl,
+                                                      DeltaArgs d) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (nl.gate[nl.parity] == 0u) return;
+  if (__hip_atomic_load(&d.bar->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  __shared__ uint32_t lds[kWave + 1];
+  if (!delta_resort<policy_pos_t<M>>(d, lds)) return;
+  list_copy_starts(nl);
+  const unsigned nt = tile_count<TX, TY, TZ, M::DIM>(a.g);
+  uint32_t longest = 0;
+  for (unsigned t = blockIdx.x; t < nt; t += gridDim.x) {
+    longest = max(longest, list_build_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(t, nt), smem));
+    __syncthreads();  // every wave is done with the tile's LDS
+  }
+  list_longest(nl, longest);
+  grid_barrier_exit(d.bar);
 }
 
 // One timestep of the step through the lists.  An agent whose list
@@ -872,13 +925,6 @@ inline void tile_budget(float occupancy, int *cap, size_t *smem, float slack = 2
   *smem = head + (size_t) *cap * M::NS * sizeof(F);
 }
 
-template <int TX, int TY, int TZ, int DIM>
-inline unsigned tile_count(const GridView &g) {
-  unsigned n = (unsigned) (((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY));
-  if (DIM == 3) n *= (unsigned) ((g.n[2] + TZ - 1) / TZ);
-  return n;
-}
-
 // Launch tiled_step for a tile shape.
 template <class M, int TX, int TY, int TZ, int RM>
 inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy, int block,
@@ -894,11 +940,29 @@ inline void launch_tiled(typename M::Args a, const GridView &g, float occupancy,
 // (mode 2) for a tile shape; same tiles and LDS budget as tiled_step.
 template <class M, int TX, int TY, int TZ, int RM>
 inline void launch_list(typename M::Args a, const GridView &g, float occupancy, int block,
-                        hipStream_t stream, float slack, const NListArgs &nl, int mode) {
+                        hipStream_t stream, float slack, const NListArgs &nl, int mode,
+                        const DeltaArgs *delta) {
   size_t smem = 0;
   tile_budget<M, TX, TY, TZ, RM>(occupancy, &a.cap, &smem, slack);
   const int threads = block > 256 ? 256 : block;
   const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
+  if (mode == 3) {
+    // half of what the CUs hold at once, a multiple of the 8 XCDs: every
+    // block is resident, so the grid barriers complete
+    if (!delta || delta->pos_elem != (int) sizeof(policy_pos_t<M>) || !delta->launched) return;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&tiled_rebuild<M, TX, TY, TZ, RM>), 256,
+            smem) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return;
+    const unsigned grid = ((unsigned) per_cu * (unsigned) cus / 2u) & ~7u;
+    if (grid < 8u) return;
+    hipLaunchKernelGGL((tiled_rebuild<M, TX, TY, TZ, RM>), grid, 256, smem, stream, a, nl, *delta);
+    *delta->launched = 1;
+    return;
+  }
   if (mode == 1) {
     hipLaunchKernelGGL((tiled_list_build<M, TX, TY, TZ, RM>), ntiles, threads, smem, stream, a, nl);
   } else {
@@ -911,9 +975,9 @@ template <class M, int TX, int TY, int TZ>
 inline void launch_tiles(const StepLaunch &L, const typename M::Args &a) {
   const int R = L.grid.reach;
   if (L.nl_mode) {
-    if (R <= 1) launch_list<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
-    else if (R == 2) launch_list<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
-    else launch_list<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode);
+    if (R <= 1) launch_list<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode, L.delta);
+    else if (R == 2) launch_list<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode, L.delta);
+    else launch_list<M, TX, TY, TZ, 4>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack, *L.nl, L.nl_mode, L.delta);
   } else {
     if (R <= 1) launch_tiled<M, TX, TY, TZ, 1>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);
     else if (R == 2) launch_tiled<M, TX, TY, TZ, 2>(a, L.grid, L.occupancy, L.block, L.stream, L.lds_slack);

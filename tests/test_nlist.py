@@ -8,6 +8,9 @@ than skin / 2; rebuilds are decided on the device.
     walk (hip.nlist=0);
   * lists too short to hold an agent's candidates (hip.list_cap) send it
     back to the stencil walk on a fresh grid: again bit-identical;
+  * the gated re-sort and list build run as one launch with grid barriers
+    (hip.rebuild=fused, the default) or as four launches (split): the two
+    are bit-identical over runs that reuse and rebuild the lists;
   * with the default skin the lists are reused for many steps: the same
     neighbours are found (exact near test on every list entry), visited in
     the order of the last build, so results match the stencil walk to the
@@ -106,3 +109,36 @@ def test_lists_one_step_vs_oracle_after_reuse():
     for f in ("pos", "velocity"):
         tol = 1e-12 * np.maximum(np.abs(want[f]), 1.0)
         assert (np.abs(got[f] - want[f]) <= tol).all(), f
+
+
+@pytest.mark.parametrize("model,prec,params,cfg", CASES, ids=IDS)
+def test_one_launch_rebuild_equals_split(model, prec, params, cfg):
+    print(f"case nlist fused vs split rebuild {model} {prec} {cfg}", flush=True)
+    steps = (1, 3, 8, 20)
+    for skin in (1e-9, 0.05):
+        a = run(model, prec, params, steps, **dict(cfg, **{"hip.skin": skin}))
+        b = run(model, prec, params, steps, **dict(cfg, **{"hip.skin": skin, "hip.rebuild": "split"}))
+        for t, x, y in zip(steps, a, b):
+            assert field_bytes(x) == field_bytes(y), f"{model} {prec} skin {skin}: step {t} differs"
+
+
+def test_abandoned_one_launch_rebuild_recovers():
+    """A one-launch rebuild whose barrier waits run out (forced here with a
+    spin limit of 0, so every block leaves at once) reports ABL_ERR_DEVICE at
+    the next synchronisation; the runtime then rebuilds from scratch with the
+    split launches and the run continues."""
+    from openabl_amd import MEMBERS, AblError, Simulation, load_library, record_dtype
+    print("case nlist abandoned one-launch rebuild", flush=True)
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    params = boids2d_params(num_agents=4096)
+    sim = Simulation("boids2d", "f64", params=params,
+                     config={"hip.skin": 1e-9, "hip.barrier_spins": 0})
+    sim.init_agents()
+    sim.simulate(3)
+    with pytest.raises(AblError):
+        sim.synchronize()
+    sim.simulate(3)
+    sim.synchronize()
+    out = sim.download(record_dtype(MEMBERS["boids2d"], "f64"))
+    sim.close()
+    assert len(out) == 4096 and np.isfinite(out["pos"]).all()
