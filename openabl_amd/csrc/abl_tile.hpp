@@ -39,44 +39,65 @@ struct TileTables {
   uint32_t lbase[MAXROWS + 1];         // LDS record index of each halo row's first agent
   uint32_t ownbase[OWNROWS + 1];       // prefix of the tile's own agents per own row
   uint32_t ownrow[OWNROWS];            // halo row index of each own row
-  uint32_t lcell[MAXROWS * (HX + 1)];  // LDS record index of each halo cell's start
+  uint32_t lcell[MAXROWS * (HX + 1)];  // global slot of each halo cell's start
 
+  // LDS record index of halo cell hx of halo row `row` (hx = hn[0]: one past)
   __device__ __forceinline__ uint32_t cell_lds(int row, int hx) const {
-    return lcell[row * (HX + 1) + hx];
+    return lbase[row] + (lcell[row * (HX + 1) + hx] - hstart[row]);
   }
   // global slot of LDS record l in halo row r
   __device__ __forceinline__ uint32_t global_of(int r, uint32_t l) const {
     return hstart[r] + (l - lbase[r]);
   }
 
+  // Geometry of tile `tile` (row-major over the tile grid): its cells
+  // [T0, T0 + TN) and its halo [H0, H0 + HN) on each axis.
+  struct Geom {
+    int T0[3], TN[3], H0[3], HN[3];
+  };
+  __device__ static Geom geom(const GridView &g, int reach, unsigned tile) {
+    Geom G;
+    const int ntx = (g.n[0] + TX - 1) / TX;
+    const int nty = (g.n[1] + TY - 1) / TY;
+    int b[3];
+    b[0] = (int) (tile % (unsigned) ntx);
+    b[1] = (int) ((tile / (unsigned) ntx) % (unsigned) nty);
+    b[2] = DIM == 3 ? (int) (tile / ((unsigned) ntx * (unsigned) nty)) : 0;
+    const int T[3] = {TX, TY, DIM == 3 ? TZ : 1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (d < DIM) {
+        G.T0[d] = b[d] * T[d];
+        G.TN[d] = min(T[d], g.n[d] - G.T0[d]);
+        G.H0[d] = max(G.T0[d] - reach, 0);
+        G.HN[d] = min(G.T0[d] + G.TN[d] + reach, g.n[d]) - G.H0[d];
+      } else {
+        G.T0[d] = G.H0[d] = 0;
+        G.TN[d] = G.HN[d] = 1;
+      }
+    }
+    return G;
+  }
+
+  // Global offset (cell-sorted slot) of halo cell e (row-major, HN[0] + 1
+  // offsets per halo row).
+  __device__ static uint32_t halo_start(const GridView &g, const Geom &G, int e) {
+    const int w = G.HN[0] + 1;
+    const int r = e / w, hx = e % w;
+    const int hy = r % G.HN[1], hz = r / G.HN[1];
+    const uint32_t grow = ((uint32_t) (G.H0[2] + hz) * (uint32_t) g.n[1] + (uint32_t) (G.H0[1] + hy)) *
+                          (uint32_t) g.n[0];
+    return g.start[grow + G.H0[0] + hx];
+  }
+
   // Block-cooperative set-up for tile `tile` (row-major over the tile grid).
   // `cap` is the LDS record capacity; overflow is set when the halo exceeds it.
   // One round of global reads: every thread derives the tile geometry itself,
   // the cell offsets of all halo cells are read at once, and the first wave
-  // forms the per-row prefixes from them (blockDim.x >= 64).
+  // forms the per-row prefixes from them (blockDim.x >= 64).  Two barriers.
   __device__ void setup(const GridView &g, int reach, unsigned tile, int cap) {
-    int T0[3], TN[3], H0[3], HN[3];
-    {
-      const int ntx = (g.n[0] + TX - 1) / TX;
-      const int nty = (g.n[1] + TY - 1) / TY;
-      int b[3];
-      b[0] = (int) (tile % (unsigned) ntx);
-      b[1] = (int) ((tile / (unsigned) ntx) % (unsigned) nty);
-      b[2] = DIM == 3 ? (int) (tile / ((unsigned) ntx * (unsigned) nty)) : 0;
-      const int T[3] = {TX, TY, DIM == 3 ? TZ : 1};
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (d < DIM) {
-          T0[d] = b[d] * T[d];
-          TN[d] = min(T[d], g.n[d] - T0[d]);
-          H0[d] = max(T0[d] - reach, 0);
-          HN[d] = min(T0[d] + TN[d] + reach, g.n[d]) - H0[d];
-        } else {
-          T0[d] = H0[d] = 0;
-          TN[d] = HN[d] = 1;
-        }
-      }
-    }
+    const Geom G = geom(g, reach, tile);
+    const int *T0 = G.T0, *TN = G.TN, *H0 = G.H0, *HN = G.HN;
     const int nr = HN[1] * HN[2];
     const int w = HN[0] + 1;
     if (threadIdx.x == 0) {
@@ -89,15 +110,9 @@ struct TileTables {
       }
       nrows = nr;
     }
-    const int nx = g.n[0], ny = g.n[1];
-    // global offsets of every halo cell (converted to LDS indices below)
-    for (int e = threadIdx.x; e < nr * w; e += blockDim.x) {
-      const int r = e / w, hx = e % w;
-      const int hy = r % HN[1], hz = r / HN[1];
-      const uint32_t grow = ((uint32_t) (H0[2] + hz) * (uint32_t) ny + (uint32_t) (H0[1] + hy)) *
-                            (uint32_t) nx;
-      lcell[r * (HX + 1) + hx] = g.start[grow + H0[0] + hx];
-    }
+    // global offsets of every halo cell (cell_lds converts them on read)
+    for (int e = threadIdx.x; e < nr * w; e += blockDim.x)
+      lcell[(e / w) * (HX + 1) + e % w] = halo_start(g, G, e);
     __syncthreads();
     const int oy = T0[1] - H0[1], oz = T0[2] - H0[2], ox = T0[0] - H0[0];
     const int nq = TN[1] * TN[2];
@@ -171,12 +186,6 @@ struct TileTables {
         ownbase[0] = 0;
         own_total = (int) ocarry;
       }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < nr * w; e += blockDim.x) {
-      const int r = e / w, hx = e % w;
-      const int k = r * (HX + 1) + hx;
-      lcell[k] = lbase[r] + (lcell[k] - hstart[r]);
     }
     __syncthreads();
   }
@@ -424,6 +433,140 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
         c.rec[(size_t) NS * c.T->total + v] = v < DIM ? (F) kFarAway<F> : (F) 0;
     }
   }
+  __syncthreads();
+  return c;
+}
+
+// Set-up and staging of a 2-D tile whose halo has at most 16 rows, for a
+// block of 256 threads (16 groups of 16 lanes), with one barrier.  Every
+// group reads the slot ranges of the halo rows and of the tile's own rows
+// itself (lane j: row j) and forms their prefixes with 16-lane DPP scans in
+// registers, so group r starts reading halo row r's records as soon as the
+// cell offsets arrive, and own agent threadIdx.x is known at once:
+// own(k, gi) runs for it (if it exists) with its slot, before the record
+// reads, so reads it issues are in flight with them.  The LDS tables are
+// written meanwhile.
+template <class M, int TX, int TY, int TZ, int RM, class Own>
+__device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const typename M::Args &a,
+                                                                      unsigned tile,
+                                                                      unsigned char *smem,
+                                                                      Own &&own) {
+  using Ctx = TileCtx<M, TX, TY, TZ, RM>;
+  using TT = typename Ctx::TT;
+  using F = typename M::F;
+  constexpr int DIM = M::DIM;
+  constexpr int NS = M::NS;
+  static_assert(DIM == 2 && TT::MAXROWS <= 16, "row-group staging: 2-D halos of <= 16 rows");
+  constexpr int kBatch = 4;
+  Ctx c;
+  c.T = reinterpret_cast<TT *>(smem);
+  c.rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
+  c.org[0] = c.org[1] = c.org[2] = 0.0;
+  const GridView &g = a.g;
+  const typename TT::Geom G = TT::geom(g, g.reach, tile);
+  const int nr = G.HN[1];
+  const int w = G.HN[0] + 1;
+  const int nq = G.TN[1];
+  const int oy = G.T0[1] - G.H0[1], ox = G.T0[0] - G.H0[0];
+  const int lane16 = (int) (threadIdx.x & 15u), grp = (int) (threadIdx.x >> 4);
+  const uint32_t nx = (uint32_t) g.n[0];
+  // one round of global reads: row ranges (per group) and halo cell offsets
+  uint32_t a0 = 0, a1 = 0, o0 = 0, o1 = 0;
+  if (lane16 < nr) {
+    const uint32_t grow = (uint32_t) (G.H0[1] + lane16) * nx;
+    a0 = g.start[grow + G.H0[0]];
+    a1 = g.start[grow + G.H0[0] + G.HN[0]];
+  }
+  if (lane16 < nq) {
+    const uint32_t grow = (uint32_t) (G.T0[1] + lane16) * nx;
+    o0 = g.start[grow + G.T0[0]];
+    o1 = g.start[grow + G.T0[0] + G.TN[0]];
+  }
+  const int ne = nr * w;
+  const uint32_t raw = (int) threadIdx.x < ne ? TT::halo_start(g, G, (int) threadIdx.x) : 0u;
+  const uint32_t cnt = a1 - a0, incl = row16_incl_scan(cnt);
+  const uint32_t ocnt = o1 - o0, oincl = row16_incl_scan(ocnt);
+  const uint32_t total = (uint32_t) __builtin_amdgcn_readlane((int) incl, 15);
+  const uint32_t own_total = (uint32_t) __builtin_amdgcn_readlane((int) oincl, 15);
+  c.ovf = (int) total >= a.cap;
+  if constexpr (policy_relative<M>::value) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+      c.org[d] = g.lo[d] + ((double) G.T0[d] + 0.5 * (double) G.TN[d]) / g.inv_cell;
+  }
+  // own agent threadIdx.x: its own row q (the first with oincl[q] > k) and
+  // slot (the shuffles run on every lane: their sources must be active)
+  {
+    const uint32_t k = threadIdx.x;
+    int q = 0;
+    for (int j = 0; j < nq; j++) q += (uint32_t) __shfl((int) oincl, j, 16) <= k ? 1 : 0;
+    q = q < 15 ? q : 15;
+    const uint32_t base = (uint32_t) __shfl((int) (oincl - ocnt), q, 16);
+    const uint32_t start = (uint32_t) __shfl((int) o0, q, 16);
+    if (k < own_total) own((int) k, start + (k - base));
+  }
+  // halo row grp: slots [hs, hs + n), records [lb, lb + n)
+  const uint32_t hs = (uint32_t) __shfl((int) a0, grp & 15, 16);
+  const uint32_t lb = (uint32_t) __shfl((int) (incl - cnt), grp & 15, 16);
+  const uint32_t n = (uint32_t) __shfl((int) cnt, grp & 15, 16);
+  if (!c.ovf && grp < nr) {
+    for (uint32_t k0 = (uint32_t) lane16; k0 < n; k0 += kBatch * 16) {
+      F rr[kBatch][NS];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const uint32_t k = k0 + (uint32_t) (u * 16);
+        if (k < n) {
+          if constexpr (policy_relative<M>::value) M::load(a, hs + k, rr[u], c.org);
+          else M::load(a, hs + k, rr[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const uint32_t k = k0 + (uint32_t) (u * 16);
+        if (k < n) {
+#pragma unroll
+          for (int v = 0; v < NS; v++) c.rec[(size_t) NS * (lb + k) + v] = rr[u][v];
+        }
+      }
+    }
+  }
+  // the LDS tables (group 0 holds the same prefixes as every group)
+  TT &T = *c.T;
+  if ((int) threadIdx.x < ne) T.lcell[(threadIdx.x / w) * (TT::HX + 1) + threadIdx.x % w] = raw;
+  for (int e = (int) threadIdx.x + (int) blockDim.x; e < ne; e += blockDim.x)
+    T.lcell[(e / w) * (TT::HX + 1) + e % w] = TT::halo_start(g, G, e);
+  if (threadIdx.x < 16u) {
+    if (lane16 < nr) {
+      T.hstart[lane16] = a0;
+      T.lbase[lane16 + 1] = incl;
+    }
+    if (lane16 < nq) {
+      T.ownrow[lane16] = (uint32_t) (oy + lane16);
+      T.ownbase[lane16 + 1] = oincl;
+    }
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      T.t0[d] = G.T0[d];
+      T.tn[d] = G.TN[d];
+      T.h0[d] = G.H0[d];
+      T.hn[d] = G.HN[d];
+    }
+    T.nrows = nr;
+    T.lbase[0] = 0;
+    T.ownbase[0] = 0;
+    T.total = (int) total;
+    T.own_total = (int) own_total;
+    T.overflow = c.ovf ? 1 : 0;
+    // record `total`: a position no near test accepts, the list padding
+    if (!c.ovf) {
+#pragma unroll
+      for (int v = 0; v < NS; v++)
+        c.rec[(size_t) NS * total + v] = v < DIM ? (F) kFarAway<F> : (F) 0;
+    }
+  }
+  (void) ox;
   __syncthreads();
   return c;
 }
@@ -811,6 +954,7 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   extern __shared__ __align__(16) unsigned char smem[];
   using F = typename M::F;
   using Ctx = TileCtx<M, TX, TY, TZ, RM>;
+  using TT = typename Ctx::TT;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
   // buckets: list rows 0..13 (capped), 14 = overflowed list, 15 = no agent
@@ -835,8 +979,23 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
       my_row = rows[gi];
     }
   };
-  const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
-                                               [&](const Ctx &cc) { fetch(cc, (int) threadIdx.x); });
+  auto stage = [&]() {
+    if constexpr (DIM == 2 && TT::MAXROWS <= 16) {
+      if (blockDim.x == 256u)
+        return tile_stage_rows<M, TX, TY, TZ, RM>(a, tile, smem, [&](int, uint32_t gi) {
+          my_len = nl.len[gi];
+          my_row = rows[gi];
+        });
+    }
+    return tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
+                                         [&](const Ctx &cc) { fetch(cc, (int) threadIdx.x); });
+  };
+  auto c = stage();
+  // the LDS pointers again from smem: through the two staging paths the
+  // compiler loses that they point to LDS and would address it with flat
+  // instructions
+  c.T = reinterpret_cast<TT *>(smem);
+  c.rec = reinterpret_cast<F *>(smem + ((sizeof(TT) + 15) & ~(size_t) 15));
   const int own_total = uniform(c.T->own_total);
   bool stray = false;
   for (int k0 = 0; k0 < own_total; k0 += (int) blockDim.x) {
