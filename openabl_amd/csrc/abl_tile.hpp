@@ -804,19 +804,27 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
   constexpr int NS = M::NS;
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
   const int own_total = uniform(c.T->own_total);
-  for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
-    F own[NS];
-    double pd[3];
-    float u[3];
-    const uint32_t gi = tile_own<M>(a, c, k, own, pd);
-    const int3 cell = own_cell<M::DIM>(a.g, pd, u);
-    typename M::Acc acc;
-    if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
-    else M::begin(a, acc, own);
-    tile_walk<M>(a, c, cell, u, a.rc, [&](const F *nb, uint32_t) { acc.visit(a, own, nb); });
-    if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, nullptr);
-    else M::finish(a, acc, own, gi, nullptr);
-  }
+  // the agent loop is specialised on where the records are (LDS or, for an
+  // overflowing halo, global memory), so each form addresses one memory
+  auto agents = [&](auto src) {
+    constexpr bool OVF = decltype(src)::value;
+    for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
+      F own[NS];
+      double pd[3];
+      float u[3];
+      const uint32_t gi = tile_own_src<M, OVF>(a, c, k, own, pd);
+      const int3 cell = own_cell<M::DIM>(a.g, pd, u);
+      typename M::Acc acc;
+      if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
+      else M::begin(a, acc, own);
+      auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
+      tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+      if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, nullptr);
+      else M::finish(a, acc, own, gi, nullptr);
+    }
+  };
+  if (c.ovf) agents(std::true_type{});
+  else agents(std::false_type{});
 }
 
 // ---- neighbour lists -------------------------------------------------------------
