@@ -154,6 +154,7 @@ void worker_loop(abl_server *s) {
   while (!s->stop) {
     // admission: in arrival order while memory and slots allow
     std::vector<Run *> admitted;
+    bool rejected = false;
     while (!s->queue.empty() && (int) s->active.size() + (int) admitted.size() < s->max_active) {
       Run *r = s->queue.front();
       if (s->used + r->footprint > s->budget) {
@@ -162,6 +163,7 @@ void worker_loop(abl_server *s) {
           r->state = RUN_FAILED;
           r->error = "run needs more device memory than the server's budget";
           r->t_done = Clock::now();
+          rejected = true;
           continue;
         }
         break;
@@ -177,15 +179,19 @@ void worker_loop(abl_server *s) {
       }
       admitted.push_back(r);
     }
+    // a waiter may already be blocked on a run rejected here
+    if (rejected) s->cv.notify_all();
     std::vector<Run *> act = s->active;
     lk.unlock();
 
-    for (Run *r : admitted) {
-      if (start_run(r)) act.push_back(r);
-    }
     bool progressed = !admitted.empty();
     std::vector<Run *> still;
     std::vector<Run *> ended;
+    for (Run *r : admitted) {
+      // a run that failed to start frees its budget and wakes its waiter
+      if (start_run(r)) act.push_back(r);
+      else ended.push_back(r);
+    }
     for (Run *r : act) {
       if (r->state != RUN_ACTIVE) {
         ended.push_back(r);
