@@ -708,12 +708,10 @@ __device__ __forceinline__ size_t list_entry(const NListArgs &nl, uint32_t i, ui
 // Visit the records of one agent's neighbour list: eight entries per 16-byte
 // row, the next row loaded while this one is visited.  The caller supplies
 // row 0 (staged with the tile).
-// wmax (wave-uniform, >= len): no entry at or past wmax is visited, so a wave
-// whose lanes' lists are shorter than their last row stops early.
 template <class M, bool OVF, class Ctx, class Fn>
 __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &c,
                                           const NListArgs &nl, uint32_t gi, uint32_t len,
-                                          uint4 row0, uint32_t wmax, Fn &fn) {
+                                          uint4 row0, Fn &fn) {
   using F = typename M::F;
   constexpr int NS = M::NS;
   // row r of this agent: four words at lp + r * rstride
@@ -726,13 +724,10 @@ __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &
     // Two records in flight, as in tile_walk_src: the record of the next
     // entry is read from LDS before the current one is visited, so the
     // visit's branches never wait on an LDS read.
-    // (entries of a tile in LDS are record byte offsets)
     F ra[NS], rb[NS];
-    const unsigned char *recb = reinterpret_cast<const unsigned char *>(c.rec);
-    auto rd = [&](F *r, uint32_t b) {
-      const F *p = reinterpret_cast<const F *>(recb + b);
+    auto rd = [&](F *r, uint32_t l) {
 #pragma unroll
-      for (int v = 0; v < NS; v++) r[v] = p[v];
+      for (int v = 0; v < NS; v++) r[v] = c.rec[(size_t) NS * l + v];
     };
     // rows are read two ahead: row k + 1 is in m0..m3 while row k is
     // visited and row k + 2 is in flight
@@ -748,29 +743,20 @@ __device__ __forceinline__ void list_walk(const typename M::Args &a, const Ctx &
     for (uint32_t e = 0; e < len; e += 8) {
       const uint32_t *np = lp + (size_t) ((e + 16 < len ? e + 16 : e) >> 3) * rstride;
       const uint32_t n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-      // entries e + j, j < q, are visited; q < 8 only in every lane's last row
-      const uint32_t q = wmax - e;
       rd(rb, c0 >> 16);
       fn(ra);
-      if (q <= 1u) break;
       rd(ra, c1 & 0xFFFFu);
       fn(rb);
-      if (q <= 2u) break;
       rd(rb, c1 >> 16);
       fn(ra);
-      if (q <= 3u) break;
       rd(ra, c2 & 0xFFFFu);
       fn(rb);
-      if (q <= 4u) break;
       rd(rb, c2 >> 16);
       fn(ra);
-      if (q <= 5u) break;
       rd(ra, c3 & 0xFFFFu);
       fn(rb);
-      if (q <= 6u) break;
       rd(rb, c3 >> 16);
       fn(ra);
-      if (q <= 7u) break;
       // first entry of the next row (a re-read after the last row)
       rd(ra, m0 & 0xFFFFu);
       fn(rb);
@@ -848,9 +834,8 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
 // build, every pair within the radius now was within radius + skin then, so
 // walking the list (and applying the exact near test) finds exactly the
 // stencil's neighbours, in the stencil's order of the build.
-// Entries (uint16) are LDS byte offsets of the build tile's halo records
-// (halo indices when the halo overflows LDS), in rows of eight interleaved
-// over the agents (list_entry) so a wave reads them coalesced.
+// Entries are halo indices of the build's tile (uint16), in rows of eight
+// interleaved over the agents (list_entry) so a wave reads them coalesced.
 // The step kernel flags a rebuild (gate of the next timestep) when an agent
 // strays past skin / 2 or a list overflowed; the rebuild kernels of the next
 // timestep run only when their gate is set.
@@ -873,9 +858,6 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
     const uint32_t gi = tile_own<M>(a, c, k, own, pd);
     const int3 cell = own_cell<M::DIM>(a.g, pd, u);
     uint32_t cnt = 0;
-    // entries: LDS record byte offsets for a tile whose halo is in LDS (its
-    // records stay below 64 KiB), halo indices otherwise
-    const uint32_t escale = c.ovf ? 1u : (uint32_t) (NS * sizeof(F));
     tile_walk<M>(a, c, cell, u, nl.rc_list, [&](const F *nb, uint32_t l) {
       double d2 = 0.0;
 #pragma unroll
@@ -884,7 +866,7 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
         d2 += e * e;
       }
       if (d2 <= nl.r2_list) {
-        if (cnt < (uint32_t) nl.cap) nl.idx[list_entry(nl, gi, cnt)] = (uint16_t) (l * escale);
+        if (cnt < (uint32_t) nl.cap) nl.idx[list_entry(nl, gi, cnt)] = (uint16_t) l;
         cnt++;
       }
     });
@@ -893,7 +875,7 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
     if (fits) {
       // pad to a whole row of eight with the far-away record (index total)
       for (uint32_t e = cnt; e < ((cnt + 7u) & ~7u); e++)
-        nl.idx[list_entry(nl, gi, e)] = (uint16_t) ((uint32_t) c.T->total * escale);
+        nl.idx[list_entry(nl, gi, e)] = (uint16_t) c.T->total;
     }
     longest = max(longest, cnt);
 #pragma unroll
@@ -971,10 +953,9 @@ __global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListAr
 // The tile's agents are taken blockDim at a time.  Their list lengths and
 // first list rows are read together with the tile's records (one round of
 // global reads) and kept in LDS; within a pass the lanes take the agents in
-// order of list length (a counting sort in LDS), so each wave walks lists of
-// about one length instead of waiting for its longest, and it stops at its
-// longest list rather than at the end of the last row of eight.  Which lane
-// handles an agent never changes the agent's results.
+// order of their number of list rows (a counting sort in LDS), so each wave
+// walks lists of about one length instead of every wave waiting for its
+// longest.  Which lane handles an agent never changes the agent's results.
 template <class M, int TX, int TY, int TZ, int RM>
 __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list(
     typename M::Args a, NListArgs nl) {
@@ -984,8 +965,8 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   using TT = typename Ctx::TT;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  // buckets: list length 0..61 (capped), 62 = overflowed list, 63 = no agent
-  constexpr uint32_t kBuckets = 64;
+  // buckets: list rows 0..13 (capped), 14 = overflowed list, 15 = no agent
+  constexpr uint32_t kBuckets = 16;
   __shared__ uint32_t bucket[kBuckets];
   __shared__ uint16_t order[256];
   __shared__ uint16_t plen[256];
@@ -1029,7 +1010,7 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     if (k0 > 0) fetch(c, k0 + (int) threadIdx.x);
     const bool valid = k0 + (int) threadIdx.x < own_total;
     uint32_t b = kBuckets - 1;
-    if (valid) b = my_len == kListOverflow ? kBuckets - 2 : min(my_len, kBuckets - 3);
+    if (valid) b = my_len == kListOverflow ? kBuckets - 2 : min((my_len + 7u) >> 3, kBuckets - 3);
     plen[threadIdx.x] = (uint16_t) (valid ? my_len : 0u);
     prow[threadIdx.x] = my_row;
     if (threadIdx.x < kBuckets) bucket[threadIdx.x] = 0u;
@@ -1037,25 +1018,17 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     const uint32_t rank = atomicAdd(&bucket[b], 1u);
     __syncthreads();
     if (threadIdx.x < (unsigned) kWave) {
-      const uint32_t v = bucket[threadIdx.x];
-      const uint32_t incl = wave_incl_scan(v);
-      bucket[threadIdx.x] = incl - v;
+      const uint32_t v = threadIdx.x < kBuckets ? bucket[threadIdx.x] : 0u;
+      const uint32_t incl = row16_incl_scan(v);
+      if (threadIdx.x < kBuckets) bucket[threadIdx.x] = incl - v;
     }
     __syncthreads();
     order[bucket[b] + rank] = (uint16_t) threadIdx.x;
     __syncthreads();
-    const int j = order[threadIdx.x] & 255;
+    const int j = order[threadIdx.x];
     const int k = k0 + j;
     const uint32_t len = plen[j];
     const uint4 row0 = prow[j];
-    // the longest list of this wave: lanes take the pass's agents in order of
-    // length, so it is the length of the wave's last agent (a length in the
-    // capped bucket or an overflowed list turns the early stop off)
-    const int nvalid = min(own_total - k0, (int) blockDim.x);
-    const int wlast = min((int) (threadIdx.x | (unsigned) (kWave - 1)), nvalid - 1);
-    uint32_t wmax = wlast >= 0 ? (uint32_t) plen[order[wlast] & 255] : 0u;
-    wmax = (uint32_t) uniform((int) wmax);
-    if (wmax >= kBuckets - 3) wmax = 0xFFFFu;
     __syncthreads();  // the next pass rewrites order, plen, prow and bucket
     if (k >= own_total) continue;
     // the whole agent body is specialised on where the records are, so the
@@ -1070,7 +1043,7 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
       else M::begin(a, acc, own);
       if (len != kListOverflow) {
         auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
-        list_walk<M, OVF>(a, c, nl, gi, len, row0, wmax, fn);
+        list_walk<M, OVF>(a, c, nl, gi, len, row0, fn);
       } else {
         stray = true;
         float u[3];
