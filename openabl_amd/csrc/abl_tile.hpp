@@ -18,6 +18,7 @@
 // every candidate, so narrowing never changes which agents are accepted.
 #pragma once
 
+#include <atomic>
 #include <type_traits>
 
 #include "abl_device.hpp"
@@ -1117,15 +1118,25 @@ inline void launch_list(typename M::Args a, const GridView &g, float occupancy, 
     // half of what the CUs hold at once, a multiple of the 8 XCDs: every
     // block is resident, so the grid barriers complete
     if (!delta || delta->pos_elem != (int) sizeof(policy_pos_t<M>) || !delta->launched) return;
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&tiled_rebuild<M, TX, TY, TZ, RM>), 256,
-            smem) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return;
-    const unsigned grid = ((unsigned) per_cu * (unsigned) cus / 2u) & ~7u;
-    if (grid < 8u) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    // the grid for (device, dynamic LDS) of the last launch, kept: the
+    // occupancy query costs more host time than the launch
+    static std::atomic<uint64_t> cached{0};
+    const uint64_t key = ((uint64_t) (uint32_t) dev << 40) | ((uint64_t) smem << 16);
+    uint64_t got = cached.load(std::memory_order_relaxed);
+    unsigned grid = (got & ~0xFFFFull) == key ? (unsigned) (got & 0xFFFFu) : 0u;
+    if (!grid) {
+      int per_cu = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, reinterpret_cast<const void *>(&tiled_rebuild<M, TX, TY, TZ, RM>), 256,
+              smem) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return;
+      grid = std::min(((unsigned) per_cu * (unsigned) cus / 2u) & ~7u, 0xFFF8u);
+      if (grid < 8u) return;
+      cached.store(key | grid, std::memory_order_relaxed);
+    }
     hipLaunchKernelGGL((tiled_rebuild<M, TX, TY, TZ, RM>), grid, 256, smem, stream, a, nl, *delta);
     *delta->launched = 1;
     return;
