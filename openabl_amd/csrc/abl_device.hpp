@@ -106,6 +106,11 @@ struct NListArgs {
   // idx[(r * stride + i) * 8 ...], so a wave's lanes read one row coalesced
   uint16_t *idx;        // [cap / 8][stride][8]
   uint16_t *len;        // [n] entries, or kListOverflow
+  // per tile, written by the build at the slot of the tile's own agent p:
+  // the own index of the tile's p-th agent in order of list rows (<< 16) and
+  // its list length, and that agent's first list row
+  uint32_t *perm;       // [n]
+  uint4 *prow0;         // [n]
   double *pos0;         // [dim][n] positions at the build
   uint32_t *gate;       // [2] rebuild flags by timestep parity
   uint32_t *stats;      // [0]: longest list at the last build

@@ -133,6 +133,8 @@ struct AgentStore {
   // neighbour lists (abl_tile.hpp): valid for the stored order while nl_ok
   bool nl_ok = false;
   uint16_t *nl_idx = nullptr, *nl_len = nullptr;
+  uint32_t *nl_perm = nullptr;  // per own slot: the tile's agents in order of list rows
+  uint4 *nl_prow0 = nullptr;    // per own slot: that agent's first list row
   double *nl_pos0 = nullptr;
   uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list,
                                 // [4..7] grid barrier of the one-launch rebuild
@@ -241,11 +243,13 @@ static void free_store(AgentStore &s) {
   s.skey[0] = s.skey[1] = s.dirty = s.mslot = s.nmov = nullptr;
   s.mhead = s.mnext = nullptr;
   s.sorted = false;
-  void *nl_ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0, s.nl_gate};
+  void *nl_ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0, s.nl_gate, s.nl_perm, s.nl_prow0};
   for (void *p : nl_ptrs) {
     if (p) (void) hipFree(p);
   }
   s.nl_idx = s.nl_len = nullptr;
+  s.nl_perm = nullptr;
+  s.nl_prow0 = nullptr;
   s.nl_pos0 = nullptr;
   s.nl_gate = nullptr;
   s.nl_cap = 0;
@@ -1192,7 +1196,7 @@ static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
   cap = (cap + 7) & ~7;  // whole 16-byte rows of entries
   if (cap > 65528) cap = 65528;
   if (s.nl_idx && s.nl_cap >= cap && s.nl_ncap >= s.n) return ABL_OK;
-  void *ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0};
+  void *ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0, s.nl_perm, s.nl_prow0};
   HIP_TRY(hipStreamSynchronize(sim->stream));
   for (void *p : ptrs) {
     if (p) HIP_TRY(hipFree(p));
@@ -1201,6 +1205,8 @@ static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
   HIP_TRY(hipMalloc(&s.nl_idx, (size_t) cap * n * 2));
   HIP_TRY(hipMalloc(&s.nl_len, n * 2));
   HIP_TRY(hipMalloc(&s.nl_pos0, 3 * n * 8));
+  HIP_TRY(hipMalloc(&s.nl_perm, n * 4));
+  HIP_TRY(hipMalloc(&s.nl_prow0, n * 16));
   if (!s.nl_gate) {
     HIP_TRY(hipMalloc(&s.nl_gate, 8 * 4));
     HIP_TRY(hipMemset(s.nl_gate, 0, 8 * 4));
@@ -1214,6 +1220,8 @@ static NListArgs list_args(const abl_sim *sim, const AgentStore &s, double r, do
   NListArgs nl{};
   nl.idx = s.nl_idx;
   nl.len = s.nl_len;
+  nl.perm = s.nl_perm;
+  nl.prow0 = s.nl_prow0;
   nl.pos0 = s.nl_pos0;
   nl.gate = s.nl_gate;
   nl.stats = s.nl_gate + 2;

@@ -887,6 +887,41 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
       nl.skey_dst[gi] = nl.skey_src[gi];
     }
   }
+  // the order the step kernel's lanes take the tile's agents in: by list rows
+  // (a counting sort; inside a bucket the order is arbitrary, and which lane
+  // takes an agent never changes its results), stored at the slots of the
+  // own agents in own order with each agent's length and first list row
+  constexpr uint32_t kBuckets = 16;  // rows 0..14 (capped), 15 = overflowed
+  __shared__ uint32_t bucket[kBuckets];
+  auto slot_of = [&](int k) {
+    int orow;
+    uint32_t ol;
+    c.T->own(k, &orow, &ol);
+    return c.T->global_of(orow, ol);
+  };
+  auto bucket_of = [](uint32_t len) {
+    return len == kListOverflow ? kBuckets - 1 : min((len + 7u) >> 3, kBuckets - 2);
+  };
+  if (threadIdx.x < kBuckets) bucket[threadIdx.x] = 0u;
+  __syncthreads();  // also: every agent's length and list are written
+  for (int k = threadIdx.x; k < own_total; k += blockDim.x)
+    atomicAdd(&bucket[bucket_of(nl.len[slot_of(k)])], 1u);
+  __syncthreads();
+  if (threadIdx.x < (unsigned) kWave) {
+    const uint32_t v = threadIdx.x < kBuckets ? bucket[threadIdx.x] : 0u;
+    const uint32_t incl = row16_incl_scan(v);
+    if (threadIdx.x < kBuckets) bucket[threadIdx.x] = incl - v;
+  }
+  __syncthreads();
+  const uint4 *rows = reinterpret_cast<const uint4 *>(nl.idx);
+  for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
+    const uint32_t gi = slot_of(k);
+    const uint32_t len = nl.len[gi];
+    const uint32_t pos = atomicAdd(&bucket[bucket_of(len)], 1u);
+    const uint32_t gp = slot_of((int) pos);
+    nl.perm[gp] = ((uint32_t) k << 16) | len;
+    nl.prow0[gp] = rows[gi];
+  }
   return longest;
 }
 
@@ -951,12 +986,12 @@ __global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListAr
 // overflowed walks the stencil (the grid is fresh: an overflow always
 // forces a rebuild on the next timestep).
 //
-// The tile's agents are taken blockDim at a time.  Their list lengths and
-// first list rows are read together with the tile's records (one round of
-// global reads) and kept in LDS; within a pass the lanes take the agents in
-// order of their number of list rows (a counting sort in LDS), so each wave
-// walks lists of about one length instead of every wave waiting for its
-// longest.  Which lane handles an agent never changes the agent's results.
+// The tile's agents are taken blockDim at a time, in the order the list
+// build stored (by number of list rows), so each wave walks lists of about
+// one length instead of every wave waiting for its longest.  Each lane's
+// agent, list length and first list row are read together with the tile's
+// records (one round of global reads).  Which lane handles an agent never
+// changes the agent's results.
 template <class M, int TX, int TY, int TZ, int RM>
 __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list(
     typename M::Args a, NListArgs nl) {
@@ -966,17 +1001,11 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   using TT = typename Ctx::TT;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  // buckets: list rows 0..13 (capped), 14 = overflowed list, 15 = no agent
-  constexpr uint32_t kBuckets = 16;
-  __shared__ uint32_t bucket[kBuckets];
-  __shared__ uint16_t order[256];
-  __shared__ uint16_t plen[256];
-  __shared__ uint4 prow[256];
   const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
-  const uint4 *rows = reinterpret_cast<const uint4 *>(nl.idx);
-  // list length and row 0 of own agent k (the pass's agent of this thread)
-  uint32_t my_len = 0;
+  // the agent of position k of the tile's order (own index << 16 | list
+  // length) and its first list row, stored by the build at own slot k
+  uint32_t my_perm = 0;
   uint4 my_row = make_uint4(0u, 0u, 0u, 0u);
   auto fetch = [&](const Ctx &cc, int k) {
     if (k < cc.T->own_total) {
@@ -984,16 +1013,16 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
       uint32_t ol;
       cc.T->own(k, &orow, &ol);
       const uint32_t gi = cc.T->global_of(orow, ol);
-      my_len = nl.len[gi];
-      my_row = rows[gi];
+      my_perm = nl.perm[gi];
+      my_row = nl.prow0[gi];
     }
   };
   auto stage = [&]() {
     if constexpr (DIM == 2 && TT::MAXROWS <= 16) {
       if (blockDim.x == 256u)
         return tile_stage_rows<M, TX, TY, TZ, RM>(a, tile, smem, [&](int, uint32_t gi) {
-          my_len = nl.len[gi];
-          my_row = rows[gi];
+          my_perm = nl.perm[gi];
+          my_row = nl.prow0[gi];
         });
     }
     return tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
@@ -1009,29 +1038,13 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   bool stray = false;
   for (int k0 = 0; k0 < own_total; k0 += (int) blockDim.x) {
     if (k0 > 0) fetch(c, k0 + (int) threadIdx.x);
-    const bool valid = k0 + (int) threadIdx.x < own_total;
-    uint32_t b = kBuckets - 1;
-    if (valid) b = my_len == kListOverflow ? kBuckets - 2 : min((my_len + 7u) >> 3, kBuckets - 3);
-    plen[threadIdx.x] = (uint16_t) (valid ? my_len : 0u);
-    prow[threadIdx.x] = my_row;
-    if (threadIdx.x < kBuckets) bucket[threadIdx.x] = 0u;
-    __syncthreads();
-    const uint32_t rank = atomicAdd(&bucket[b], 1u);
-    __syncthreads();
-    if (threadIdx.x < (unsigned) kWave) {
-      const uint32_t v = threadIdx.x < kBuckets ? bucket[threadIdx.x] : 0u;
-      const uint32_t incl = row16_incl_scan(v);
-      if (threadIdx.x < kBuckets) bucket[threadIdx.x] = incl - v;
-    }
-    __syncthreads();
-    order[bucket[b] + rank] = (uint16_t) threadIdx.x;
-    __syncthreads();
-    const int j = order[threadIdx.x];
-    const int k = k0 + j;
-    const uint32_t len = plen[j];
-    const uint4 row0 = prow[j];
-    __syncthreads();  // the next pass rewrites order, plen, prow and bucket
-    if (k >= own_total) continue;
+    // lanes take the pass's positions of the tile's order: agents of about
+    // one list length per wave
+    if (k0 + (int) threadIdx.x >= own_total) continue;
+    const int k = (int) (my_perm >> 16);
+    const uint32_t len = my_perm & 0xFFFFu;
+    const uint4 row0 = my_row;
+
     // the whole agent body is specialised on where the records are, so the
     // LDS path never waits on global reads of the overflow path
     auto agent = [&](auto src) {
