@@ -100,6 +100,9 @@ __device__ __forceinline__ void gather_cols(const GatherCols &g, uint32_t src, u
 // Neighbour-list state of one agent type (abl_tile.hpp, tiled_list_build /
 // tiled_step_list).
 constexpr uint16_t kListOverflow = 0xFFFF;
+// lane order entry of a tile without lists: position p takes own agent p,
+// which walks the stencil
+constexpr uint32_t kOwnOrder = 0xFFFFFFFFu;
 
 struct NListArgs {
   // halo indices in rows of eight entries (16 bytes): row r of agent i is

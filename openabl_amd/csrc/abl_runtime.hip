@@ -1261,8 +1261,11 @@ static int enter_lists(abl_sim *sim, AgentStore &b, const StepDesc &st, const St
     uint32_t longest = 0;
     HIP_TRY(hipMemcpyAsync(&longest, b.nl_gate + 2, 4, hipMemcpyDeviceToHost, sim->stream));
     HIP_TRY(hipStreamSynchronize(sim->stream));
-    if ((int) longest <= b.nl_cap || b.nl_cap >= 65528 || sim->list_cap) break;
-    rc = reserve_lists(sim, b, (int) (1.25 * longest) + 16);
+    // grow to the longest list, but to at most four times the estimate: an
+    // agent in a dense cluster with a longer list walks the stencil instead
+    const int grow = std::min((int) (1.25 * longest) + 16, 4 * want);
+    if ((int) longest <= b.nl_cap || grow <= b.nl_cap || b.nl_cap >= 65528 || sim->list_cap) break;
+    rc = reserve_lists(sim, b, grow);
     if (rc) return rc;
   }
   b.nl_ok = true;

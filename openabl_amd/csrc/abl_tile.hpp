@@ -80,12 +80,13 @@ struct TileTables {
     return G;
   }
 
-  // Global offset (cell-sorted slot) of halo cell e (row-major, HN[0] + 1
-  // offsets per halo row).
-  __device__ static uint32_t halo_start(const GridView &g, const Geom &G, int e) {
-    const int w = G.HN[0] + 1;
-    const int r = e / w, hx = e % w;
-    const int hy = r % G.HN[1], hz = r / G.HN[1];
+  // Global offset (cell-sorted slot) of cell hx (0..HN[0]) of halo row r.
+  __device__ static uint32_t halo_start(const GridView &g, const Geom &G, int r, int hx) {
+    int hy = r, hz = 0;
+    if constexpr (DIM == 3) {
+      hy = r % G.HN[1];
+      hz = r / G.HN[1];
+    }
     const uint32_t grow = ((uint32_t) (G.H0[2] + hz) * (uint32_t) g.n[1] + (uint32_t) (G.H0[1] + hy)) *
                           (uint32_t) g.n[0];
     return g.start[grow + G.H0[0] + hx];
@@ -111,9 +112,12 @@ struct TileTables {
       }
       nrows = nr;
     }
-    // global offsets of every halo cell (cell_lds converts them on read)
-    for (int e = threadIdx.x; e < nr * w; e += blockDim.x)
-      lcell[(e / w) * (HX + 1) + e % w] = halo_start(g, G, e);
+    // global offsets of every halo cell (cell_lds converts them on read);
+    // entry e of the table is cell e % (HX + 1) of halo row e / (HX + 1)
+    for (int e = threadIdx.x; e < nr * (HX + 1); e += blockDim.x) {
+      const int r = e / (HX + 1), hx = e % (HX + 1);
+      if (hx < w) lcell[e] = halo_start(g, G, r, hx);
+    }
     __syncthreads();
     const int oy = T0[1] - H0[1], oz = T0[2] - H0[2], ox = T0[0] - H0[0];
     const int nq = TN[1] * TN[2];
@@ -129,7 +133,7 @@ struct TileTables {
           cnt = lcell[lane * (HX + 1) + HN[0]] - a;
         } else if (lane >= 16 && lane < 16 + nq) {
           const int q = lane - 16;
-          const int r = (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]);
+          const int r = DIM == 3 ? (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]) : oy + q;
           ownrow[q] = (uint32_t) r;
           cnt = lcell[r * (HX + 1) + ox + TN[0]] - lcell[r * (HX + 1) + ox];
         }
@@ -175,7 +179,7 @@ struct TileTables {
         const int q = q0 + lane;
         uint32_t cnt = 0;
         if (q < nq) {
-          const int r = (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]);
+          const int r = DIM == 3 ? (oz + q / TN[1]) * HN[1] + (oy + q % TN[1]) : oy + q;
           ownrow[q] = (uint32_t) r;
           cnt = lcell[r * (HX + 1) + ox + TN[0]] - lcell[r * (HX + 1) + ox];
         }
@@ -444,8 +448,8 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage(const typename 
 // itself (lane j: row j) and forms their prefixes with 16-lane DPP scans in
 // registers, so group r starts reading halo row r's records as soon as the
 // cell offsets arrive, and own agent threadIdx.x is known at once:
-// own(k, gi) runs for it (if it exists) with its slot, before the record
-// reads, so reads it issues are in flight with them.  The LDS tables are
+// own(k, gi, l) runs for it (if it exists) with its slot and LDS record
+// index, before the record reads, so reads it issues are in flight with them.  The LDS tables are
 // written meanwhile.
 template <class M, int TX, int TY, int TZ, int RM, class Own>
 __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const typename M::Args &a,
@@ -483,8 +487,13 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const type
     o0 = g.start[grow + G.T0[0]];
     o1 = g.start[grow + G.T0[0] + G.TN[0]];
   }
-  const int ne = nr * w;
-  const uint32_t raw = (int) threadIdx.x < ne ? TT::halo_start(g, G, (int) threadIdx.x) : 0u;
+  // lcell entry e: cell e % (HX + 1) of halo row e / (HX + 1) (a constant
+  // divisor: no integer division at run time)
+  constexpr int LW = TT::HX + 1;
+  const int ne = nr * LW;
+  const int er = (int) threadIdx.x / LW, ex = (int) threadIdx.x % LW;
+  const bool eok = (int) threadIdx.x < ne && ex < w;
+  const uint32_t raw = eok ? TT::halo_start(g, G, er, ex) : 0u;
   const uint32_t cnt = a1 - a0, incl = row16_incl_scan(cnt);
   const uint32_t ocnt = o1 - o0, oincl = row16_incl_scan(ocnt);
   const uint32_t total = (uint32_t) __builtin_amdgcn_readlane((int) incl, 15);
@@ -504,7 +513,14 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const type
     q = q < 15 ? q : 15;
     const uint32_t base = (uint32_t) __shfl((int) (oincl - ocnt), q, 16);
     const uint32_t start = (uint32_t) __shfl((int) o0, q, 16);
-    if (k < own_total) own((int) k, start + (k - base));
+    // own row q is halo row oy + q: its first slot and LDS record
+    const int hr = oy + q < 15 ? oy + q : 15;
+    const uint32_t hs_r = (uint32_t) __shfl((int) a0, hr, 16);
+    const uint32_t lb_r = (uint32_t) __shfl((int) (incl - cnt), hr, 16);
+    if (k < own_total) {
+      const uint32_t gi = start + (k - base);
+      own((int) k, gi, lb_r + (gi - hs_r));
+    }
   }
   // halo row grp: slots [hs, hs + n), records [lb, lb + n)
   const uint32_t hs = (uint32_t) __shfl((int) a0, grp & 15, 16);
@@ -533,9 +549,10 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const type
   }
   // the LDS tables (group 0 holds the same prefixes as every group)
   TT &T = *c.T;
-  if ((int) threadIdx.x < ne) T.lcell[(threadIdx.x / w) * (TT::HX + 1) + threadIdx.x % w] = raw;
-  for (int e = (int) threadIdx.x + (int) blockDim.x; e < ne; e += blockDim.x)
-    T.lcell[(e / w) * (TT::HX + 1) + e % w] = TT::halo_start(g, G, e);
+  if (eok) T.lcell[threadIdx.x] = raw;
+  for (int e = (int) threadIdx.x + (int) blockDim.x; e < ne; e += blockDim.x) {
+    if (e % LW < w) T.lcell[e] = TT::halo_start(g, G, e / LW, e % LW);
+  }
   if (threadIdx.x < 16u) {
     if (lane16 < nr) {
       T.hstart[lane16] = a0;
@@ -586,17 +603,13 @@ __device__ __forceinline__ void tile_load(const typename M::Args &a, const Ctx &
   else M::load(a, j, r);
 }
 
-// Own agent k of the tile: global slot, record and FP64 position; OVF: the
-// records are in global memory (the halo did not fit LDS).
+// The own agent at global slot gi (LDS record ol): its record and FP64
+// position; OVF: the records are in global memory (the halo did not fit LDS).
 template <class M, bool OVF, class Ctx>
-__device__ __forceinline__ uint32_t tile_own_src(const typename M::Args &a, const Ctx &c, int k,
-                                                 typename M::F *own, double *pd) {
+__device__ __forceinline__ void tile_own_at(const typename M::Args &a, const Ctx &c, uint32_t gi,
+                                            uint32_t ol, typename M::F *own, double *pd) {
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  int orow;
-  uint32_t ol;
-  c.T->own(k, &orow, &ol);
-  const uint32_t gi = c.T->global_of(orow, ol);
   if constexpr (!OVF) {
 #pragma unroll
     for (int v = 0; v < NS; v++) own[v] = c.rec[(size_t) NS * ol + v];
@@ -611,6 +624,18 @@ __device__ __forceinline__ uint32_t tile_own_src(const typename M::Args &a, cons
 #pragma unroll
     for (int d = 0; d < DIM; d++) pd[d] = (double) own[d];
   }
+}
+
+// Own agent k of the tile (global slot gi, LDS record ol): record and FP64
+// position; returns gi.
+template <class M, bool OVF, class Ctx>
+__device__ __forceinline__ uint32_t tile_own_src(const typename M::Args &a, const Ctx &c, int k,
+                                                 typename M::F *own, double *pd) {
+  int orow;
+  uint32_t ol;
+  c.T->own(k, &orow, &ol);
+  const uint32_t gi = c.T->global_of(orow, ol);
+  tile_own_at<M, OVF>(a, c, gi, ol, own, pd);
   return gi;
 }
 
@@ -852,6 +877,9 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
   const auto c = tile_stage<M, TX, TY, TZ, RM>(a, tile, smem);
   uint32_t longest = 0;
   const int own_total = uniform(c.T->own_total);
+  // list entries and the lane order hold 16-bit halo and own indices: a tile
+  // with more records keeps no lists (its agents walk the stencil)
+  const bool tile_lists = uniform(c.T->total) < (int) kListOverflow;
   for (int k = threadIdx.x; k < own_total; k += blockDim.x) {
     F own[NS];
     double pd[3];
@@ -871,14 +899,14 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
         cnt++;
       }
     });
-    const bool fits = cnt <= (uint32_t) nl.cap && cnt < kListOverflow;
+    const bool fits = tile_lists && cnt <= (uint32_t) nl.cap && cnt < kListOverflow;
     nl.len[gi] = fits ? (uint16_t) cnt : kListOverflow;
     if (fits) {
       // pad to a whole row of eight with the far-away record (index total)
       for (uint32_t e = cnt; e < ((cnt + 7u) & ~7u); e++)
         nl.idx[list_entry(nl, gi, e)] = (uint16_t) c.T->total;
     }
-    longest = max(longest, cnt);
+    if (tile_lists) longest = max(longest, cnt);
 #pragma unroll
     for (int d = 0; d < DIM; d++) nl.pos0[(size_t) d * nl.n + gi] = pd[d];
     if (nl.copy.ncols) {
@@ -902,6 +930,10 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
   auto bucket_of = [](uint32_t len) {
     return len == kListOverflow ? kBuckets - 1 : min((len + 7u) >> 3, kBuckets - 2);
   };
+  if (!tile_lists) {
+    for (int k = threadIdx.x; k < own_total; k += blockDim.x) nl.perm[slot_of(k)] = kOwnOrder;
+    return longest;
+  }
   if (threadIdx.x < kBuckets) bucket[threadIdx.x] = 0u;
   __syncthreads();  // also: every agent's length and list are written
   for (int k = threadIdx.x; k < own_total; k += blockDim.x)
@@ -1003,6 +1035,8 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
   constexpr int NS = M::NS;
   const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
+  // own agent k (< 256) of the row-group staging: its slot and LDS record
+  __shared__ uint2 ownmap[256];
   // the agent of position k of the tile's order (own index << 16 | list
   // length) and its first list row, stored by the build at own slot k
   uint32_t my_perm = 0;
@@ -1017,13 +1051,17 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
       my_row = nl.prow0[gi];
     }
   };
+  // row-group staging (which fills ownmap) when the tile shape allows it
+  const bool mapped = DIM == 2 && TT::MAXROWS <= 16 && blockDim.x == 256u;
   auto stage = [&]() {
     if constexpr (DIM == 2 && TT::MAXROWS <= 16) {
-      if (blockDim.x == 256u)
-        return tile_stage_rows<M, TX, TY, TZ, RM>(a, tile, smem, [&](int, uint32_t gi) {
-          my_perm = nl.perm[gi];
-          my_row = nl.prow0[gi];
-        });
+      if (mapped)
+        return tile_stage_rows<M, TX, TY, TZ, RM>(a, tile, smem,
+                                                  [&](int k, uint32_t gi, uint32_t ol) {
+                                                    ownmap[k] = make_uint2(gi, ol);
+                                                    my_perm = nl.perm[gi];
+                                                    my_row = nl.prow0[gi];
+                                                  });
     }
     return tile_stage<M, TX, TY, TZ, RM>(a, tile, smem,
                                          [&](const Ctx &cc) { fetch(cc, (int) threadIdx.x); });
@@ -1041,8 +1079,9 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     // lanes take the pass's positions of the tile's order: agents of about
     // one list length per wave
     if (k0 + (int) threadIdx.x >= own_total) continue;
-    const int k = (int) (my_perm >> 16);
-    const uint32_t len = my_perm & 0xFFFFu;
+    const bool own_order = my_perm == kOwnOrder;
+    const int k = own_order ? k0 + (int) threadIdx.x : (int) (my_perm >> 16);
+    const uint32_t len = own_order ? (uint32_t) kListOverflow : my_perm & 0xFFFFu;
     const uint4 row0 = my_row;
 
     // the whole agent body is specialised on where the records are, so the
@@ -1051,7 +1090,14 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
       constexpr bool OVF = decltype(src)::value;
       F own[NS];
       double pd[3];
-      const uint32_t gi = tile_own_src<M, OVF>(a, c, k, own, pd);
+      uint32_t gi;
+      if (mapped && k < 256) {
+        const uint2 m = ownmap[k];
+        gi = m.x;
+        tile_own_at<M, OVF>(a, c, gi, m.y, own, pd);
+      } else {
+        gi = tile_own_src<M, OVF>(a, c, k, own, pd);
+      }
       typename M::Acc acc;
       if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
       else M::begin(a, acc, own);
@@ -1089,7 +1135,8 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
 }
 
 // Host side: LDS record capacity (`slack` times the expected halo population
-// plus 64, within a 64 KiB block budget) and the block's dynamic LDS bytes.
+// plus 64, within a 64 KiB block budget less the 2 KiB of static LDS the list
+// step kernel holds) and the block's dynamic LDS bytes.
 // Less slack leaves room for more resident blocks per CU; a tile whose halo
 // does not fit reads from global memory instead (same results).
 template <class M, int TX, int TY, int TZ, int RM>
@@ -1099,7 +1146,7 @@ inline void tile_budget(float occupancy, int *cap, size_t *smem, float slack = 2
   const size_t head = (sizeof(TT) + 15) & ~(size_t) 15;
   const double halo_cells = (double) (TX + 2 * RM) * (TY + 2 * RM) *
                             (M::DIM == 3 ? (double) (TZ + 2 * RM) : 1.0);
-  const size_t budget = 64 * 1024 - head;
+  const size_t budget = 62 * 1024 - head;
   const int max_cap = (int) (budget / (M::NS * sizeof(F)));
   const int want = (int) ((double) slack * halo_cells * (double) occupancy) + 64;
   *cap = want < max_cap ? want : max_cap;

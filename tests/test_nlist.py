@@ -142,3 +142,18 @@ def test_abandoned_one_launch_rebuild_recovers():
     out = sim.download(record_dtype(MEMBERS["boids2d"], "f64"))
     sim.close()
     assert len(out) == 4096 and np.isfinite(out["pos"]).all()
+
+
+def test_tile_beyond_16bit_indices_walks_stencil():
+    """A tile whose halo holds 65,535 records or more keeps no lists (list
+    entries and the lane order hold 16-bit indices) and its agents walk the
+    stencil: 66,000 boids in a 40 x 40 box fall in one tile, and the run is
+    bit-identical to hip.nlist=0."""
+    print("case nlist tile beyond 16-bit indices", flush=True)
+    params = boids2d_params(num_agents=66000, agent_density=66000 / 1600.0)
+    assert params["max_pos"] == 40.0
+    steps = (1, 2)
+    a = run("boids2d", "f64", params, steps, **{"hip.cell_scale": 2, "hip.cell_inflate": 1})
+    b = run("boids2d", "f64", params, steps, **{"hip.cell_scale": 2, "hip.nlist": 0})
+    for t, x, y in zip(steps, a, b):
+        assert field_bytes(x) == field_bytes(y), t
