@@ -132,7 +132,9 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
  * entries per agent; a longer list falls back to the stencil walk),
  * "hip.cell_inflate" (cell widening factor; list mode widens by 1 + skin),
  * "hip.rebuild" ("fused": a list rebuild's re-sort and build run as one
- * launch with grid barriers; "split": four launches), "hip.barrier_spins"
+ * launch with grid barriers, while this is the process's only live
+ * simulation -- with several, each uses the split form, since two such
+ * launches could wait on each other; "split": four launches), "hip.barrier_spins"
  * (spin limit of those barriers; a launch that runs out abandons the rebuild,
  * the next synchronize or download returns ABL_ERR_DEVICE and the runtime
  * rebuilds from scratch with the split launches from then on),

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -743,6 +744,13 @@ static double step_radius(const abl_sim *sim, const StepDesc &st) {
 #define CHECK_SIM(sim) \
   if (!(sim)) return fail(ABL_ERR_ARG, "null simulation handle")
 
+// Live simulations of the process.  The one-launch list rebuild waits on
+// grid barriers that need every block of its launch resident; two such
+// launches from different simulations (streams) could each hold part of the
+// device and wait on the other until the spin limit, so the one-launch form
+// runs only while a single simulation is live.
+static std::atomic<int> g_live_sims{0};
+
 extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out) {
   if (!program || !out) return fail(ABL_ERR_ARG, "null argument");
   *out = nullptr;
@@ -772,12 +780,14 @@ extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out)
     delete sim;
     return fail(ABL_ERR_DEVICE, msg);
   }
+  g_live_sims.fetch_add(1, std::memory_order_relaxed);
   *out = sim;
   return ABL_OK;
 }
 
 extern "C" int abl_hip_destroy(abl_sim *sim) {
   CHECK_SIM(sim);
+  g_live_sims.fetch_sub(1, std::memory_order_relaxed);
   (void) hipStreamSynchronize(sim->stream);
   for (AgentStore &s : sim->types) free_store(s);
   if (sim->staging) (void) hipFree(sim->staging);
@@ -1292,7 +1302,8 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
   // program's launch function can run it; else four gated launches
   DeltaArgs d{};
   int launched = 0;
-  if (sim->fused_rebuild && b.n > 0 && b.nmov) {
+  if (sim->fused_rebuild && g_live_sims.load(std::memory_order_relaxed) == 1 && b.n > 0 &&
+      b.nmov) {
     const int pc = b.pos_col;
     d.px = col_cur(b, pc);
     d.py = col_cur(b, pc + 1);
