@@ -157,3 +157,31 @@ def test_tile_beyond_16bit_indices_walks_stencil():
     b = run("boids2d", "f64", params, steps, **{"hip.cell_scale": 2, "hip.nlist": 0})
     for t, x, y in zip(steps, a, b):
         assert field_bytes(x) == field_bytes(y), t
+
+
+def test_two_live_simulations_equal_alone():
+    """Two live handles stepping in turn (each on its own stream, so their
+    launches overlap on the device) rebuild their lists with the four-launch
+    form; every timestep still rebuilds (vanishing skin) and both runs equal
+    the same runs made alone, bit for bit."""
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    print("case nlist two live simulations", flush=True)
+    cfg = {"hip.cell_scale": 2, "hip.skin": 1e-9}
+    pa, pb = boids2d_params(num_agents=20000), boids2d_params(num_agents=12000)
+    alone = [run("boids2d", "f64", p, (10,), **cfg)[0] for p in (pa, pb)]
+    dt = record_dtype(MEMBERS["boids2d"], "f64")
+    lib = load_library()
+    lib.abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    a = Simulation("boids2d", "f64", params=pa, config=cfg)
+    a.init_agents()
+    lib.abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    b = Simulation("boids2d", "f64", params=pb, config=cfg)
+    b.init_agents()
+    for _ in range(5):
+        a.simulate(2)
+        b.simulate(2)
+    got = [a.download(dt), b.download(dt)]
+    a.close()
+    b.close()
+    for x, y in zip(got, alone):
+        assert field_bytes(x) == field_bytes(y)
