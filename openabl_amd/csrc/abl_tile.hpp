@@ -887,6 +887,18 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
     const uint32_t gi = tile_own<M>(a, c, k, own, pd);
     const int3 cell = own_cell<M::DIM>(a.g, pd, u);
     uint32_t cnt = 0;
+    // the current row of eight entries, stored as one 16-byte write when full
+    uint64_t lo = 0, hi = 0;
+    auto put = [&](uint32_t e, uint32_t l) {
+      const uint64_t v = (uint64_t) (l & 0xFFFFu) << (16u * (e & 3u));
+      if (e < 4u) lo |= v;
+      else hi |= v;
+    };
+    auto store_row = [&](uint32_t r) {
+      reinterpret_cast<uint4 *>(nl.idx)[(size_t) r * (size_t) nl.stride + gi] =
+          make_uint4((uint32_t) lo, (uint32_t) (lo >> 32), (uint32_t) hi, (uint32_t) (hi >> 32));
+      lo = hi = 0;
+    };
     tile_walk<M>(a, c, cell, u, nl.rc_list, [&](const F *nb, uint32_t l) {
       double d2 = 0.0;
 #pragma unroll
@@ -895,16 +907,19 @@ __device__ __forceinline__ uint32_t list_build_tile(const typename M::Args &a, c
         d2 += e * e;
       }
       if (d2 <= nl.r2_list) {
-        if (cnt < (uint32_t) nl.cap) nl.idx[list_entry(nl, gi, cnt)] = (uint16_t) l;
+        if (cnt < (uint32_t) nl.cap) {
+          put(cnt & 7u, l);
+          if ((cnt & 7u) == 7u) store_row(cnt >> 3);
+        }
         cnt++;
       }
     });
     const bool fits = tile_lists && cnt <= (uint32_t) nl.cap && cnt < kListOverflow;
     nl.len[gi] = fits ? (uint16_t) cnt : kListOverflow;
-    if (fits) {
-      // pad to a whole row of eight with the far-away record (index total)
-      for (uint32_t e = cnt; e < ((cnt + 7u) & ~7u); e++)
-        nl.idx[list_entry(nl, gi, e)] = (uint16_t) c.T->total;
+    if (fits && (cnt & 7u)) {
+      // pad the last row with the far-away record (index total)
+      for (uint32_t e = cnt & 7u; e < 8u; e++) put(e, (uint32_t) c.T->total);
+      store_row(cnt >> 3);
     }
     if (tile_lists) longest = max(longest, cnt);
 #pragma unroll
