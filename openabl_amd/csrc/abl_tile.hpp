@@ -509,7 +509,9 @@ __device__ __forceinline__ TileCtx<M, TX, TY, TZ, RM> tile_stage_rows(const type
   {
     const uint32_t k = threadIdx.x;
     int q = 0;
-    for (int j = 0; j < nq; j++) q += (uint32_t) __shfl((int) oincl, j, 16) <= k ? 1 : 0;
+    // (every 16-lane group holds the same prefixes: lane j's is row j's)
+    for (int j = 0; j < nq; j++)
+      q += (uint32_t) __builtin_amdgcn_readlane((int) oincl, j) <= k ? 1 : 0;
     q = q < 15 ? q : 15;
     const uint32_t base = (uint32_t) __shfl((int) (oincl - ocnt), q, 16);
     const uint32_t start = (uint32_t) __shfl((int) o0, q, 16);
