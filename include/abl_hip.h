@@ -131,13 +131,17 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
  * "hip.skin" (list skin as a fraction of the radius), "hip.list_cap" (list
  * entries per agent; a longer list falls back to the stencil walk),
  * "hip.cell_inflate" (cell widening factor; list mode widens by 1 + skin),
- * "hip.rebuild" ("fused": a list rebuild's re-sort and build run as one
- * launch with grid barriers, while this is the process's only live
- * simulation -- with several, each uses the split form, since two such
- * launches could wait on each other; "split": four launches), "hip.barrier_spins"
- * (spin limit of those barriers; a launch that runs out abandons the rebuild,
- * the next synchronize or download returns ABL_ERR_DEVICE and the runtime
- * rebuilds from scratch with the split launches from then on),
+ * "hip.rebuild" ("fused", the default: a list rebuild's re-sort and build
+ * run as one launch whose phases are ordered by a work queue -- blocks claim
+ * items in order and wait only for items already claimed by running blocks,
+ * so every wait ends whatever the launch size and whatever else runs on the
+ * device; "split": four launches; the two are bit-identical),
+ * "hip.rebuild_blocks" (blocks of that launch, 0 = as many as the CUs hold at
+ * once; larger values only add blocks that find the queue empty),
+ * "hip.barrier_spins" (spin limit of one phase wait, far above any phase; a
+ * limit forced below a phase's duration abandons the rebuild, the next
+ * synchronize or download returns ABL_ERR_DEVICE and the runtime rebuilds
+ * from scratch with the split launches from then on),
  * "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
  * the counter-based generator random() uses inside steps), "hip.log_csv"
  * (file log_csv rows are appended to). */
@@ -177,10 +181,17 @@ int abl_hip_simulate(abl_sim *sim, int timesteps);
 int abl_hip_synchronize(abl_sim *sim);
 /* Non-blocking: *done = 1 when all work queued on the sim has finished. */
 int abl_hip_query(abl_sim *sim, int *done);
-/* Device bytes a simulation of `program` holding `agents` agents over `cells`
- * grid cells would need (admission control for serving). */
-int abl_hip_estimate_footprint(const char *program, int precision, size_t agents, size_t cells,
-                               size_t *bytes);
+/* An upper bound of the device bytes a simulation of `program` with these
+ * parameters (-P overrides; the rest at their defaults) would hold at any
+ * time while it has at most `agents` agents of each type: columns, ids, grid
+ * and sort scratch (at least `cells` cells, and the grid of the finest cell
+ * size), neighbour lists at the largest capacity the runtime grows them to,
+ * add()/removeCurrent() flags and newborn staging, upload/download staging.
+ * Admission control for serving; abl_hip_footprint() of a live simulation
+ * never exceeds it. */
+int abl_hip_estimate_footprint(const char *program, int precision, int num_params,
+                               const char *const *param_names, const double *param_values,
+                               size_t agents, size_t cells, size_t *bytes);
 
 /* Parity probe: the (centre id, neighbour id) pairs that step function
  * `step`'s near loop accepts on the current state, sorted by centre then
@@ -256,6 +267,9 @@ typedef struct {
   uint64_t rng_state[2];           /* generator state the run's init starts from */
   size_t agents_hint;              /* most agents the run holds (admission) */
   size_t cells_hint;               /* grid cells of its environment (admission) */
+  int num_config;                  /* -C overrides (abl_hip_set_config), or 0 */
+  const char *const *config_keys;
+  const char *const *config_values;
 } abl_run_request;
 
 enum { ABL_RUN_QUEUED = 0, ABL_RUN_ACTIVE = 1, ABL_RUN_DONE = 2, ABL_RUN_FAILED = 3 };

@@ -262,16 +262,31 @@ __device__ __forceinline__ void place_item(int s, const uint32_t *__restrict__ k
 // ---- the gated re-sort in one launch --------------------------------------------
 // In neighbour-list mode the re-sort runs only on timesteps whose gate the
 // previous step set.  Its kernels (bin, scan, place, then the list build)
-// can run as one launch separated by grid barriers, so a timestep without a
-// rebuild costs one launch that returns at once instead of four.
+// can run as one launch, so a timestep without a rebuild costs one launch that
+// returns at once instead of four.
+//
+// The phases are ordered by a work queue, not by grid barriers.  The launch's
+// work is a sequence of items -- bin chunks, scan tiles, place chunks, then
+// list-build tiles -- that blocks claim one at a time with an atomic counter,
+// in increasing order.  Before it works on an item of phase p a block waits
+// until every item of phase p - 1 is finished.  Those items were all claimed
+// before this one, so each is held by a block that is already running and
+// stays resident until it is done; phase-0 items wait for nothing.  By
+// induction over the phases every wait ends whether or not the launch's
+// blocks fit on the device at once, and whatever else runs beside it (other
+// simulations' launches on other streams).  A block that finds the queue
+// empty leaves; the last one out resets the queue for the next launch.
 
-// Grid-barrier state of one agent type (device memory, zero between launches).
-struct GridBarrier {
-  uint32_t count;   // arrivals of this launch
-  uint32_t exited;  // blocks done with this launch
-  uint32_t error;   // set when a barrier wait ran out: the launch was abandoned
+// Work-queue state of one agent type (device memory, zero between launches).
+struct RebuildQueue {
+  uint32_t next;     // items claimed
+  uint32_t done[4];  // items finished, per phase
+  uint32_t exited;   // blocks that found the queue empty
+  uint32_t error;    // a wait ran past its spin limit (a safety net: by the
+                     // argument above no wait can, unless the limit is forced)
   uint32_t pad;
 };
+static_assert(sizeof(RebuildQueue) == 32, "the queue occupies eight words of the gate block");
 
 // Everything the re-sort phases read and write.
 struct DeltaArgs {
@@ -290,75 +305,103 @@ struct DeltaArgs {
   GatherCols gc;              // current -> other buffers
   const int32_t *id_in;
   int32_t *id_out;
-  GridBarrier *bar;
-  uint32_t spins;             // spin limit of a barrier wait (kBarrierSpins)
+  RebuildQueue *q;
+  uint32_t spins;             // spin limit of one phase wait (kPhaseSpins)
+  unsigned blocks;            // launch size (0: as many blocks as the CUs hold at once)
   int *launched;              // host: set to 1 when the launch function launched it
 };
 
-// Spin limit of one barrier wait (each spin sleeps ~0.25 us): far above any
-// phase of a rebuild, so it only ends a launch whose blocks are not all
-// resident.
-constexpr uint32_t kBarrierSpins = 1u << 20;
+// Spin limit of one phase wait (each spin sleeps ~0.1 us plus a load): far
+// above any phase of a rebuild.
+constexpr uint32_t kPhaseSpins = 1u << 22;
 
-// Barrier across all blocks of a launch whose blocks are all resident at once
-// (the grid is at most half of what the CUs hold).  Thread 0 of each block
-// arrives with an agent-scope atomic and waits with s_sleep.  A wait that runs
-// out sets bar->error and returns false, as does any wait once the error is
-// set: every block then leaves the launch, so no wave waits for ever.
-__device__ __forceinline__ bool grid_barrier(GridBarrier *bar, uint32_t target, uint32_t spins) {
-  __shared__ int ok;
+// Items per phase of a launch with blocks of 256 threads.
+struct QueuePlan {
+  uint32_t end[4];  // exclusive end of each phase's items
+  __device__ __host__ int phase_of(uint32_t item) const {
+    return item < end[0] ? 0 : item < end[1] ? 1 : item < end[2] ? 2 : 3;
+  }
+  __device__ __host__ uint32_t first(int p) const { return p == 0 ? 0u : end[p - 1]; }
+  __device__ __host__ uint32_t items(int p) const { return end[p] - first(p); }
+};
+
+// Claim the block's next item (block-uniform result).
+__device__ __forceinline__ uint32_t queue_claim(RebuildQueue *q) {
+  __shared__ uint32_t claimed;
+  __syncthreads();  // every thread has read the previous claim
+  if (threadIdx.x == 0)
+    claimed = __hip_atomic_fetch_add(&q->next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
+  return claimed;
+}
+
+// Wait until `count` items of phase p are finished.  Thread 0 waits with an
+// acquire load; after the block barrier every thread acquires at agent scope
+// too, so its reads see the finished phase's writes from every XCD.  Returns
+// false when the wait ran past the spin limit or another block's did.
+__device__ __forceinline__ bool queue_wait(RebuildQueue *q, int p, uint32_t count, uint32_t spins) {
+  __shared__ int ok;
   if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     int good = 1;
     for (uint32_t spin = 0;; spin++) {
-      if (__hip_atomic_load(&bar->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (__hip_atomic_load(&q->done[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= count) break;
       if (spin >= spins ||
-          __hip_atomic_load(&bar->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-        __hip_atomic_fetch_or(&bar->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_load(&q->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        __hip_atomic_fetch_or(&q->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         good = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(4);
     }
-    __threadfence();
     ok = good;
   }
   __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return ok != 0;
 }
 
-// The last block out of a launch zeroes the arrival counters for the next one.
-__device__ __forceinline__ void grid_barrier_exit(GridBarrier *bar) {
+// Mark the block's item of phase p finished: every thread releases its writes
+// at agent scope before the block barrier, then thread 0 counts the item.
+__device__ __forceinline__ void queue_finish(RebuildQueue *q, int p) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add(&q->done[p], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The last block out of a launch zeroes the queue for the next one.  Every
+// block leaves only after finishing the items it claimed, so when the last
+// one leaves every item is done.
+__device__ __forceinline__ void queue_exit(RebuildQueue *q) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t e =
-        __hip_atomic_fetch_add(&bar->exited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&q->exited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (e == gridDim.x - 1u) {
-      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar->exited, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&q->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int p = 0; p < 4; p++)
+        __hip_atomic_store(&q->done[p], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&q->exited, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
-// Phases 1-3 of the gated re-sort (grid-stride; blockDim.x == kScanBlock).
-// Returns false when a barrier was abandoned.
+// One item of the re-sort's phases 0-2 (blockDim.x == kScanBlock == 256):
+// bin chunk, scan tile, or place chunk `k` of its phase.
 template <typename P>
-__device__ __forceinline__ bool delta_resort(const DeltaArgs &d, uint32_t *lds) {
-  const int stride = (int) (gridDim.x * blockDim.x);
-  const int first = (int) (blockIdx.x * blockDim.x + threadIdx.x);
-  for (int i = first; i < d.n; i += stride)
-    bin_delta_item<P>(i, (const P *) d.px, (const P *) d.py, (const P *) d.pz, d.g, d.skey, d.key,
-                      d.count, d.bsum, d.ml, d.gen, d.n);
-  if (!grid_barrier(d.bar, gridDim.x, d.spins)) return false;
-  for (int t = (int) blockIdx.x; t < d.nscan; t += (int) gridDim.x)
-    scan_apply_tile(t, d.nscan, d.count, d.bsum, d.start_new, d.ncells, lds);
-  if (!grid_barrier(d.bar, 2u * gridDim.x, d.spins)) return false;
-  for (int s = first; s < d.n; s += stride)
-    place_item(s, d.key, d.skey, d.skey_out, d.start_old, d.start_new, d.ml, d.gen, d.gc, d.id_in,
-               d.id_out, d.n);
-  return grid_barrier(d.bar, 3u * gridDim.x, d.spins);
+__device__ __forceinline__ void delta_item(const DeltaArgs &d, int phase, uint32_t k,
+                                           uint32_t *lds) {
+  if (phase == 0) {
+    const int i = (int) (k * blockDim.x + threadIdx.x);
+    if (i < d.n)
+      bin_delta_item<P>(i, (const P *) d.px, (const P *) d.py, (const P *) d.pz, d.g, d.skey,
+                        d.key, d.count, d.bsum, d.ml, d.gen, d.n);
+  } else if (phase == 1) {
+    scan_apply_tile((int) k, d.nscan, d.count, d.bsum, d.start_new, d.ncells, lds);
+  } else {
+    place_item((int) (k * blockDim.x + threadIdx.x), d.key, d.skey, d.skey_out, d.start_old,
+               d.start_new, d.ml, d.gen, d.gc, d.id_in, d.id_out, d.n);
+  }
 }
 
 static __global__ void k_place(const uint32_t *__restrict__ key, const uint32_t *__restrict__ skey_in,

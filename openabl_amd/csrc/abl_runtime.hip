@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,26 @@ static int fail(int code, const std::string &msg) {
   } while (0)
 
 extern "C" const char *abl_hip_last_error(void) { return g_last_error.c_str(); }
+
+// Device allocations of one call, freed when the call returns (error paths
+// included).
+struct ScopedDevice {
+  std::vector<void *> ptrs;
+  ScopedDevice() = default;
+  ScopedDevice(const ScopedDevice &) = delete;
+  ScopedDevice &operator=(const ScopedDevice &) = delete;
+  ~ScopedDevice() {
+    for (void *p : ptrs) (void) hipFree(p);
+  }
+  template <typename T>
+  hipError_t alloc(T **p, size_t bytes) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e == hipSuccess) ptrs.push_back(q);
+    *p = (T *) q;
+    return e;
+  }
+};
 
 // ---- program registry -----------------------------------------------------
 
@@ -138,7 +159,7 @@ struct AgentStore {
   uint4 *nl_prow0 = nullptr;    // per own slot: that agent's first list row
   double *nl_pos0 = nullptr;
   uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list,
-                                // [4..7] grid barrier of the one-launch rebuild
+                                // [8..15] work queue of the one-launch rebuild
   int nl_cap = 0;               // entries per agent
   size_t nl_ncap = 0;           // agents allocated
   // run-time add/remove: ids stop being 0..n-1 once agents die or are born
@@ -149,6 +170,9 @@ struct AgentStore {
 };
 
 }  // namespace
+
+// Default list skin, as a fraction of the near radius.
+constexpr double kDefaultSkin = 0.05;
 
 struct abl_sim {
   const ProgramDesc *prog = nullptr;
@@ -164,11 +188,12 @@ struct abl_sim {
   bool grid_delta = true;  // "hip.grid": delta re-sort when the stored order allows it
   float lds_slack = 2.0f;  // "hip.lds_slack": LDS tile records per expected halo agent
   bool nlist = true;       // "hip.nlist": neighbour lists for steps that support them
-  double skin = 0.05;      // "hip.skin": list skin as a fraction of the near radius
+  double skin = kDefaultSkin;  // "hip.skin": list skin as a fraction of the near radius
   int list_cap = 0;        // "hip.list_cap": entries per list (0: sized from the data)
   double cell_inflate = 0.0;  // "hip.cell_inflate": cell widening factor (0: automatic)
   bool fused_rebuild = true;  // "hip.rebuild": gated re-sort + list build in one launch
-  uint32_t barrier_spins = kBarrierSpins;  // "hip.barrier_spins": its grid barriers' spin limit
+  uint32_t barrier_spins = kPhaseSpins;   // "hip.barrier_spins": spin limit of its phase waits
+  unsigned rebuild_blocks = 0;  // "hip.rebuild_blocks": its launch size (0: resident capacity)
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -266,42 +291,56 @@ static void free_store(AgentStore &s) {
   s.grid_valid = false;
 }
 
-static size_t store_bytes(const AgentStore &s) {
+// Bytes of one record in every column of a type (the device element sizes).
+static size_t col_bytes(const AgentStore &s) {
   size_t b = 0;
-  for (const Column &c : s.cols) b += 2 * s.cap * (size_t) c.elem;
+  for (const Column &c : s.cols) b += (size_t) c.elem;
+  return b;
+}
+
+// Neighbour-list bytes per agent besides the entries: length (2), lane order
+// (4), first row (16), build positions (3 x 8).
+constexpr size_t kListMetaBytes = 2 + 4 + 16 + 24;
+
+// Device bytes one agent type holds: columns (two buffers), ids, grid and
+// sort scratch, neighbour lists, newborn staging.
+static size_t store_bytes(const AgentStore &s) {
+  size_t b = 2 * s.cap * col_bytes(s);
   b += 2 * s.cap * 4;                      // ids
   b += s.cap * (4 + 4 + 4 + 8 + 4 + 8);    // key, tmp_idx, tmp_id, skey x2, mover slot, next
-  b += s.cells_cap * (4 + 8 + 4 + 8) + 4096 * 4;  // count, start x2, dirty, head; bsum
+  const size_t tiles = (s.cells_cap + 2047) / 2048;
+  b += s.cells_cap * (4 + 8 + 4 + 8) + std::max<size_t>(tiles, 1) * 4;  // count, start x2, dirty, head; bsum
+  if (s.nl_idx) b += s.nl_ncap * ((size_t) s.nl_cap * 2 + kListMetaBytes) + 16 * 4;
+  b += s.bcap * col_bytes(s);              // newborn staging
   return b;
 }
 
 static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
   if (n <= s.cap) return ABL_OK;
   size_t cap = std::max<size_t>(n, 1024);
+  // a column grown to `cap` records, keeping its first n: the new buffer
+  // replaces the old one only once the copy is done
+  auto grow = [&](void **slot, size_t elem) -> int {
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, cap * elem));
+    hipError_t e = hipSuccess;
+    if (*slot && s.n) e = hipMemcpyAsync(p, *slot, s.n * elem, hipMemcpyDeviceToDevice, sim->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+    if (e != hipSuccess) {
+      (void) hipFree(p);
+      return fail(ABL_ERR_DEVICE, std::string("column copy: ") + hipGetErrorString(e));
+    }
+    if (*slot) HIP_TRY(hipFree(*slot));
+    *slot = p;
+    return ABL_OK;
+  };
   for (int b = 0; b < 2; b++) {
     for (size_t c = 0; c < s.cols.size(); c++) {
-      void *p = nullptr;
-      HIP_TRY(hipMalloc(&p, cap * (size_t) s.cols[c].elem));
-      if (s.mem[b][c] && s.n) {
-        HIP_TRY(hipMemcpyAsync(p, s.mem[b][c], s.n * (size_t) s.cols[c].elem,
-                               hipMemcpyDeviceToDevice, sim->stream));
-      }
-      if (s.mem[b][c]) {
-        HIP_TRY(hipStreamSynchronize(sim->stream));
-        HIP_TRY(hipFree(s.mem[b][c]));
-      }
-      s.mem[b][c] = p;
+      int rc = grow(&s.mem[b][c], (size_t) s.cols[c].elem);
+      if (rc) return rc;
     }
-    int32_t *p = nullptr;
-    HIP_TRY(hipMalloc(&p, cap * 4));
-    if (s.id[b] && s.n) {
-      HIP_TRY(hipMemcpyAsync(p, s.id[b], s.n * 4, hipMemcpyDeviceToDevice, sim->stream));
-    }
-    if (s.id[b]) {
-      HIP_TRY(hipStreamSynchronize(sim->stream));
-      HIP_TRY(hipFree(s.id[b]));
-    }
-    s.id[b] = p;
+    int rc = grow((void **) &s.id[b], 4);
+    if (rc) return rc;
   }
   void *ptrs[] = {s.key, s.tmp_idx, s.tmp_id, s.skey[0], s.skey[1], s.mslot, s.mnext};
   for (void *p : ptrs) {
@@ -744,12 +783,6 @@ static double step_radius(const abl_sim *sim, const StepDesc &st) {
 #define CHECK_SIM(sim) \
   if (!(sim)) return fail(ABL_ERR_ARG, "null simulation handle")
 
-// Live simulations of the process.  The one-launch list rebuild waits on
-// grid barriers that need every block of its launch resident; two such
-// launches from different simulations (streams) could each hold part of the
-// device and wait on the other until the spin limit, so the one-launch form
-// runs only while a single simulation is live.
-static std::atomic<int> g_live_sims{0};
 
 extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out) {
   if (!program || !out) return fail(ABL_ERR_ARG, "null argument");
@@ -780,14 +813,12 @@ extern "C" int abl_hip_create(const char *program, int precision, abl_sim **out)
     delete sim;
     return fail(ABL_ERR_DEVICE, msg);
   }
-  g_live_sims.fetch_add(1, std::memory_order_relaxed);
   *out = sim;
   return ABL_OK;
 }
 
 extern "C" int abl_hip_destroy(abl_sim *sim) {
   CHECK_SIM(sim);
-  g_live_sims.fetch_sub(1, std::memory_order_relaxed);
   (void) hipStreamSynchronize(sim->stream);
   for (AgentStore &s : sim->types) free_store(s);
   if (sim->staging) (void) hipFree(sim->staging);
@@ -893,7 +924,7 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
       for (AgentStore &s : sim->types) {
         if (s.nl_gate) {
           HIP_TRY(hipStreamSynchronize(sim->stream));
-          HIP_TRY(hipMemset(s.nl_gate + 4, 0, 4 * 4));
+          HIP_TRY(hipMemset(s.nl_gate + 8, 0, 8 * 4));
         }
       }
       sim->fused_rebuild = true;
@@ -903,8 +934,12 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     else return fail(ABL_ERR_ARG, "hip.rebuild must be fused or split");
   } else if (k == "hip.barrier_spins") {
     const long long sp = std::atoll(value);
-    if (sp < 0 || sp > (long long) kBarrierSpins) return fail(ABL_ERR_ARG, "hip.barrier_spins out of range");
+    if (sp < 0 || sp > (long long) kPhaseSpins) return fail(ABL_ERR_ARG, "hip.barrier_spins out of range");
     sim->barrier_spins = (uint32_t) sp;
+  } else if (k == "hip.rebuild_blocks") {
+    const long long nb = std::atoll(value);
+    if (nb < 0 || nb > 0xFFF8) return fail(ABL_ERR_ARG, "hip.rebuild_blocks must be 0..65528");
+    sim->rebuild_blocks = (unsigned) nb;
   } else if (k == "hip.seed") {
     sim->seed = std::strtoull(value, nullptr, 0);
   } else if (k == "hip.log_csv") {
@@ -1218,8 +1253,8 @@ static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
   HIP_TRY(hipMalloc(&s.nl_perm, n * 4));
   HIP_TRY(hipMalloc(&s.nl_prow0, n * 16));
   if (!s.nl_gate) {
-    HIP_TRY(hipMalloc(&s.nl_gate, 8 * 4));
-    HIP_TRY(hipMemset(s.nl_gate, 0, 8 * 4));
+    HIP_TRY(hipMalloc(&s.nl_gate, 16 * 4));
+    HIP_TRY(hipMemset(s.nl_gate, 0, 16 * 4));
   }
   s.nl_cap = cap;
   s.nl_ncap = n;
@@ -1302,8 +1337,7 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
   // program's launch function can run it; else four gated launches
   DeltaArgs d{};
   int launched = 0;
-  if (sim->fused_rebuild && g_live_sims.load(std::memory_order_relaxed) == 1 && b.n > 0 &&
-      b.nmov) {
+  if (sim->fused_rebuild && b.n > 0 && b.nmov) {
     const int pc = b.pos_col;
     d.px = col_cur(b, pc);
     d.py = col_cur(b, pc + 1);
@@ -1325,8 +1359,9 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
     d.gc = placed;
     d.id_in = b.id[b.id_cur];
     d.id_out = b.id[1 - b.id_cur];
-    d.bar = reinterpret_cast<GridBarrier *>(b.nl_gate + 4);
+    d.q = reinterpret_cast<RebuildQueue *>(b.nl_gate + 8);
     d.spins = sim->barrier_spins;
+    d.blocks = sim->rebuild_blocks;
     d.launched = &launched;
   } else {
     launch_delta(sim, b, gate, start_cur, start_other, placed);
@@ -1364,23 +1399,27 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
   return ABL_OK;
 }
 
-// A one-launch rebuild that abandoned a grid barrier (its blocks were not all
-// resident) left its agent type half re-sorted: report it at the next
-// synchronisation, rebuild from scratch from then on and stop using it.
+// A one-launch rebuild whose phase wait ran past its spin limit left its
+// agent type half re-sorted.  Every wait of the work queue ends by
+// construction (abl_pipeline.hpp), so this happens only with a spin limit
+// forced far below a phase's duration ("hip.barrier_spins"); it is reported at
+// the next synchronisation as a device error, and the runtime rebuilds from
+// scratch with the split launches from then on.
 static int check_rebuild(abl_sim *sim) {
   for (AgentStore &s : sim->types) {
     if (!s.nl_gate) continue;
     uint32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, s.nl_gate + 6, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&err, s.nl_gate + 8 + offsetof(RebuildQueue, error) / 4, 4,
+                      hipMemcpyDeviceToHost));
     if (err) {
-      HIP_TRY(hipMemset(s.nl_gate + 4, 0, 4 * 4));
+      HIP_TRY(hipMemset(s.nl_gate + 8, 0, 8 * 4));
       sim->fused_rebuild = false;
       for (AgentStore &t : sim->types) {
         t.nl_ok = false;
         t.grid_valid = false;
         t.sorted = false;
       }
-      return fail(ABL_ERR_DEVICE, "one-launch neighbour-list rebuild abandoned a grid barrier; "
+      return fail(ABL_ERR_DEVICE, "one-launch neighbour-list rebuild ran past its spin limit; "
                                   "results since the last synchronisation are invalid");
     }
   }
@@ -1575,23 +1614,67 @@ extern "C" int abl_hip_query(abl_sim *sim, int *done) {
 
 // Device bytes a simulation of `program` with `agents` agents in total would
 // hold (columns, ids, grid scratch, staging), for admission control.
-extern "C" int abl_hip_estimate_footprint(const char *program, int precision, size_t agents,
+extern "C" int abl_hip_estimate_footprint(const char *program, int precision, int num_params,
+                                          const char *const *param_names,
+                                          const double *param_values, size_t agents,
                                           size_t cells, size_t *bytes) {
-  if (!program || !bytes) return fail(ABL_ERR_ARG, "null argument");
-  const ProgramDesc *prog = nullptr;
-  for (int i = 0; programs()[i]; i++) {
-    if (std::strcmp(programs()[i]->name, program) == 0) prog = programs()[i];
-  }
+  if (!program || !bytes || num_params < 0 || (num_params && (!param_names || !param_values)))
+    return fail(ABL_ERR_ARG, "null argument");
+  const ProgramDesc *prog = find_program(program);
   if (!prog) return fail(ABL_ERR_ARG, std::string("unknown program ") + program);
-  size_t per_agent = 0;
-  for (const AgentDesc &d : prog->agents) {
-    AgentStore s;
-    layout_agent(s, d, precision);
-    size_t cols = 0;
-    for (const Column &c : s.cols) cols += (size_t) c.elem;
-    per_agent = std::max(per_agent, 2 * cols + 8 + 12 + s.aos_size + 2 * cols);
+  std::vector<double> params;
+  for (const ParamDesc &p : prog->params) params.push_back(p.value);
+  for (int k = 0; k < num_params; k++) {
+    size_t p = 0;
+    while (p < prog->params.size() && std::strcmp(prog->params[p].name, param_names[k]) != 0) p++;
+    if (p == prog->params.size())
+      return fail(ABL_ERR_ARG, std::string("Unknown parameter \"") + param_names[k] + "\"");
+    params[p] = param_values[k];
   }
-  *bytes = agents * per_agent + cells * 8 * prog->agents.size() + (1u << 20);
+  double lo[3], hi[3], gran = 0.0;
+  prog->env(params.data(), lo, hi, &gran);
+  bool adds = false, removes = false;
+  for (const StepDesc &st : prog->steps) {
+    adds = adds || st.adds;
+    removes = removes || st.removes;
+  }
+  size_t b = 1u << 20;  // events, gate blocks, allocator rounding
+  for (size_t t = 0; t < prog->agents.size(); t++) {
+    AgentStore s;
+    layout_agent(s, prog->agents[t], precision);
+    const size_t cb = col_bytes(s);
+    // columns, ids, sort scratch, upload/download staging
+    size_t per_agent = 2 * cb + 8 + 32 + s.aos_size;
+    if (adds) per_agent += 2 * cb;      // a parent's newborn staging, the child store's growth
+    if (adds || removes) per_agent += 2 * 14;  // flags, their scan, birth lists (2 per agent)
+    // the grid of this type, at the finest cell size a near loop over it uses
+    size_t ncells = cells;
+    double list_entries = 0.0;
+    for (const StepDesc &st : prog->steps) {
+      if (st.near_agent != (int) t || s.dim == 0 || !(gran > 0.0)) continue;
+      double r = 0.0;
+      for (size_t p = 0; p < prog->params.size(); p++)
+        if (std::strcmp(prog->params[p].name, st.radius_param) == 0) r = st.radius_scale * params[p];
+      const double cell = gran / 4.0;  // hip.cell_scale at most 4
+      double nc = 1.0, vol = 1.0;
+      for (int d = 0; d < s.dim; d++) {
+        nc *= std::ceil((hi[d] - lo[d]) / cell) + 1.0;
+        vol *= std::max(hi[d] - lo[d], cell);
+      }
+      ncells = std::max(ncells, (size_t) std::min(nc, 4e9));
+      if (st.nlist && st.near_agent == st.agent && !st.removes && !st.adds) {
+        // the list capacity enter_lists can reach at the default skin: the
+        // estimate of a uniform population grown at most four times
+        const double rl = r * (1.0 + kDefaultSkin);
+        const double ball = s.dim == 3 ? 4.0 / 3.0 * M_PI * rl * rl * rl : M_PI * rl * rl;
+        const double want = 1.5 * ball * (double) agents / vol + 16.0;
+        list_entries = std::max(list_entries, std::min(4.0 * want + 8.0, 65528.0));
+      }
+    }
+    if (list_entries > 0.0) per_agent += (size_t) (2.0 * list_entries) + kListMetaBytes;
+    b += agents * per_agent + (ncells + 1) * (4 + 8 + 4 + 8 + 4);
+  }
+  *bytes = b;
   return ABL_OK;
 }
 
@@ -1600,6 +1683,7 @@ extern "C" int abl_hip_footprint(abl_sim *sim, size_t *bytes) {
   if (!bytes) return fail(ABL_ERR_ARG, "null argument");
   size_t b = sim->staging_cap;
   for (const AgentStore &s : sim->types) b += store_bytes(s);
+  if (sim->dead) b += sim->fcap * (1 + 1 + 4 + 4 + 4) + 4 + (sim->fcap / 2048 + 1) * 4;
   *bytes = b;
   return ABL_OK;
 }
@@ -1632,8 +1716,9 @@ extern "C" int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t
   const float t = sqrtf_le_threshold(r);
   uint32_t *cnt = nullptr, *off = nullptr;
   int64_t *dp = nullptr;
-  HIP_TRY(hipMalloc(&cnt, (size_t) n * 4));
-  HIP_TRY(hipMalloc(&off, (size_t) n * 4));
+  ScopedDevice scratch;
+  HIP_TRY(scratch.alloc(&cnt, (size_t) n * 4));
+  HIP_TRY(scratch.alloc(&off, (size_t) n * 4));
   const int pa = a.pos_col, pb = b.pos_col;
   auto launch = [&](uint32_t *counts, const uint32_t *offsets, int64_t *out) {
     if (a.cols[pa].elem == 8) {
@@ -1665,7 +1750,7 @@ extern "C" int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t
   if (count) *count = total;
   if (pairs && total > 0) {
     HIP_TRY(hipMemcpyAsync(off, ho.data(), (size_t) n * 4, hipMemcpyHostToDevice, sim->stream));
-    HIP_TRY(hipMalloc(&dp, total * 16));
+    HIP_TRY(scratch.alloc(&dp, total * 16));
     launch(nullptr, off, dp);
     std::vector<int64_t> hp(total * 2);
     HIP_TRY(hipMemcpyAsync(hp.data(), dp, total * 16, hipMemcpyDeviceToHost, sim->stream));
@@ -1678,11 +1763,8 @@ extern "C" int abl_hip_neighbours(abl_sim *sim, int step, int64_t *pairs, size_t
       pairs[2 * k] = v[k].first;
       pairs[2 * k + 1] = v[k].second;
     }
-    HIP_TRY(hipFree(dp));
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipFree(cnt));
-  HIP_TRY(hipFree(off));
   return ABL_OK;
 }
 
@@ -1713,7 +1795,8 @@ extern "C" int abl_hip_reduce_sum(abl_sim *sim, int t, const char *member, doubl
   const int blocks = std::max(1, std::min(1024, (n + 255) / 256));
   const int kind = s.desc->members[s.cols[c0].member].kind;
   void *partial = nullptr;
-  HIP_TRY(hipMalloc(&partial, (size_t) blocks * 8));
+  ScopedDevice scratch;
+  HIP_TRY(scratch.alloc(&partial, (size_t) blocks * 8));
   for (int k = 0; k < nc; k++) {
     const void *col = col_cur(s, c0 + k);
     if (n == 0) {
@@ -1745,7 +1828,6 @@ extern "C" int abl_hip_reduce_sum(abl_sim *sim, int t, const char *member, doubl
     }
     out[k] = (kind == K_BOOL || kind == K_INT) ? (double) iacc : acc;
   }
-  HIP_TRY(hipFree(partial));
   return ABL_OK;
 }
 
@@ -1765,7 +1847,8 @@ extern "C" int abl_hip_reduce_count_eq(abl_sim *sim, int t, const char *member, 
   const int blocks = std::max(1, std::min(1024, (n + 255) / 256));
   const int kind = s.desc->members[s.cols[c0].member].kind;
   unsigned long long *partial = nullptr;
-  HIP_TRY(hipMalloc(&partial, (size_t) blocks * 8));
+  ScopedDevice scratch;
+  HIP_TRY(scratch.alloc(&partial, (size_t) blocks * 8));
   const void *col = col_cur(s, c0);
   if (kind == K_BOOL) {
     hipLaunchKernelGGL(k_reduce_count_eq<uint8_t>, blocks, 256, 0, sim->stream,
@@ -1784,7 +1867,6 @@ extern "C" int abl_hip_reduce_count_eq(abl_sim *sim, int t, const char *member, 
   std::vector<unsigned long long> h((size_t) blocks);
   HIP_TRY(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, sim->stream));
   HIP_TRY(hipStreamSynchronize(sim->stream));
-  HIP_TRY(hipFree(partial));
   unsigned long long acc = 0;
   for (unsigned long long v : h) acc += v;
   *out = (size_t) acc;

@@ -2,16 +2,24 @@
 //
 // Requests (a program, its parameters, a timestep count, the generator
 // state its initialisation starts from) arrive at any time through
-// abl_server_submit.  A worker thread admits queued requests while the
-// estimated device footprint of the active runs stays within the HBM budget
-// and the active count within `max_active`, builds each admitted run's initial
-// state on the host (the generator is process-wide, so initialisations are
-// serialised here), and then advances every active run `chunk_steps`
-// timesteps at a time on that run's own stream.  Work of different runs
-// overlaps on the device; a run that finishes is downloaded, its memory is
-// released, and the next queued request can take its place.  Each run is
-// computed exactly as it would be alone: runs share no device state, and
-// agent order inside a run depends only on that run's positions and ids.
+// abl_server_submit.  Three threads serve them:
+//   issuer    admits queued requests in arrival order while the estimated
+//             device footprint of the admitted runs stays within the HBM
+//             budget and their count within `max_active`, and advances every
+//             ready run `chunk_steps` timesteps at a time on that run's own
+//             stream (a launch queue, never a wait), so runs overlap on the
+//             device;
+//   starter   builds each admitted run's simulation and initial state on the
+//             host and uploads it (the generator is process-wide, so the
+//             initialisations are serialised on this one thread);
+//   finisher  downloads each finished run (reference array order), keeps its
+//             records and log, and releases its device memory.
+// Host initialisation and download therefore never hold up issuing work to
+// the runs already on the device.  A run's state, error text, timestamps
+// and results are published together under the server's mutex, which is
+// also how every reader takes them.  Each run is computed exactly as it
+// would be alone: runs share no device state, and agent order inside a run
+// depends only on that run's positions and ids.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -31,7 +39,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-enum { RUN_QUEUED = 0, RUN_ACTIVE = 1, RUN_DONE = 2, RUN_FAILED = 3 };
+// Public states (abl_hip.h) plus the two hand-over states, reported as ACTIVE.
+enum { RUN_QUEUED = 0, RUN_ACTIVE = 1, RUN_DONE = 2, RUN_FAILED = 3, RUN_STARTING = 4,
+       RUN_FINISHING = 5 };
 
 struct Run {
   uint64_t id = 0;
@@ -39,13 +49,14 @@ struct Run {
   int precision = 0;
   std::vector<std::string> names;
   std::vector<double> values;
+  std::vector<std::pair<std::string, std::string>> config;
   int timesteps = 0;
   uint64_t rng[2] = {0xdeadbeefULL, 0xbeefdeadULL};
   size_t footprint = 0;
-  std::atomic<int> state{RUN_QUEUED};
-  std::string error;
-  abl_sim *sim = nullptr;
-  int issued = 0;  // timesteps handed to the device so far
+  int state = RUN_QUEUED;  // guarded by abl_server::mu
+  std::string error;       // guarded by abl_server::mu
+  abl_sim *sim = nullptr;  // owned by the thread that holds the run
+  int issued = 0;          // timesteps handed to the device so far (issuer)
   size_t agents = 0;
   Clock::time_point t_submit, t_admit, t_done;
   std::vector<std::vector<char>> results;  // per agent type, generated-struct records
@@ -57,14 +68,18 @@ struct Run {
 
 struct abl_server {
   std::mutex mu;
-  std::condition_variable cv;
-  std::deque<Run *> queue;
-  std::vector<Run *> active;
+  std::condition_variable cv;       // waiters: a run ended
+  std::condition_variable work_cv;  // issuer, starter, finisher: work arrived
+  std::deque<Run *> queue;          // submitted, not admitted
+  std::deque<Run *> to_start;       // admitted, waiting for the starter
+  std::vector<Run *> ready;         // started, held by the issuer
+  std::deque<Run *> to_finish;      // all timesteps done, waiting for the finisher
   std::map<uint64_t, Run *> runs;
-  std::thread worker;
+  std::thread issuer, starter, finisher;
   bool stop = false;
   size_t budget = 0, used = 0, peak = 0;
   int max_active = 8;
+  int admitted = 0;  // runs holding budget (starting, ready or finishing)
   int chunk = 20;
   uint64_t next_id = 1;
   double agent_steps = 0.0;
@@ -82,31 +97,54 @@ int serve_fail(int code, const std::string &msg) {
   return code;
 }
 
-void fail_run(Run *r, const std::string &what) {
-  r->state = RUN_FAILED;
-  r->error = what + ": " + abl_hip_last_error();
+// End a run (caller holds s->mu): publish state, error and time together,
+// release its budget, wake its waiters.
+void end_run_locked(abl_server *s, Run *r, int state, const std::string &error) {
+  r->error = error;
+  r->t_done = Clock::now();
+  r->state = state;
+  if (state == RUN_DONE) {
+    s->agent_steps += (double) r->agents * (double) r->timesteps;
+    s->t_last_done = r->t_done;
+  }
+  s->used -= r->footprint;
+  s->admitted -= 1;
+  s->cv.notify_all();
+  s->work_cv.notify_all();
+}
+
+// A failed run's message and device memory (outside the lock, by the thread
+// that holds the run).
+std::string drop_sim(Run *r, const std::string &what) {
+  std::string msg = what + ": " + abl_hip_last_error();
   if (r->sim) {
     abl_hip_destroy(r->sim);
     r->sim = nullptr;
   }
-  r->t_done = Clock::now();
+  return msg;
 }
 
-// Build the run's simulation and initial state (worker thread only).
-bool start_run(Run *r) {
+// Build the run's simulation and initial state (starter thread).
+bool start_run(Run *r, std::string *err) {
   if (abl_hip_create(r->program.c_str(), r->precision, &r->sim)) {
-    fail_run(r, "create");
+    *err = drop_sim(r, "create");
     return false;
   }
   for (size_t k = 0; k < r->names.size(); k++) {
     if (abl_hip_set_param(r->sim, r->names[k].c_str(), r->values[k])) {
-      fail_run(r, "parameter " + r->names[k]);
+      *err = drop_sim(r, "parameter " + r->names[k]);
+      return false;
+    }
+  }
+  for (const auto &kv : r->config) {
+    if (abl_hip_set_config(r->sim, kv.first.c_str(), kv.second.c_str())) {
+      *err = drop_sim(r, "config " + kv.first);
       return false;
     }
   }
   abl_rng_seed(r->rng[0], r->rng[1]);
   if (abl_hip_init_agents(r->sim)) {
-    fail_run(r, "init");
+    *err = drop_sim(r, "init");
     return false;
   }
   r->agents = 0;
@@ -118,20 +156,25 @@ bool start_run(Run *r) {
   return true;
 }
 
-bool finish_run(Run *r) {
+// Download the results and release the device memory (finisher thread).
+bool finish_run(Run *r, std::string *err) {
+  if (abl_hip_synchronize(r->sim)) {
+    *err = drop_sim(r, "synchronize");
+    return false;
+  }
   const int nt = abl_hip_agent_type_count(r->sim);
   r->results.resize((size_t) nt);
   r->counts.resize((size_t) nt);
   for (int t = 0; t < nt; t++) {
     size_t n = 0;
     if (abl_hip_count(r->sim, t, &n)) {
-      fail_run(r, "count");
+      *err = drop_sim(r, "count");
       return false;
     }
     const size_t rec = abl_hip_agent_size(r->sim, t);
     r->results[(size_t) t].resize(std::max<size_t>(n, 1) * rec);
     if (abl_hip_download(r->sim, t, r->results[(size_t) t].data(), n, &r->counts[(size_t) t])) {
-      fail_run(r, "download");
+      *err = drop_sim(r, "download");
       return false;
     }
   }
@@ -144,102 +187,131 @@ bool finish_run(Run *r) {
   }
   abl_hip_destroy(r->sim);
   r->sim = nullptr;
-  r->state = RUN_DONE;
-  r->t_done = Clock::now();
   return true;
 }
 
-void worker_loop(abl_server *s) {
-  std::unique_lock<std::mutex> lk(s->mu);
-  while (!s->stop) {
-    // admission: in arrival order while memory and slots allow
-    std::vector<Run *> admitted;
-    bool rejected = false;
-    while (!s->queue.empty() && (int) s->active.size() + (int) admitted.size() < s->max_active) {
-      Run *r = s->queue.front();
-      if (s->used + r->footprint > s->budget) {
-        if (s->active.empty() && admitted.empty() && r->footprint > s->budget) {
-          s->queue.pop_front();  // can never fit
-          r->state = RUN_FAILED;
-          r->error = "run needs more device memory than the server's budget";
-          r->t_done = Clock::now();
-          rejected = true;
-          continue;
-        }
-        break;
-      }
-      s->queue.pop_front();
-      s->used += r->footprint;
-      s->peak = std::max(s->peak, s->used);
-      r->state = RUN_ACTIVE;
-      r->t_admit = Clock::now();
-      if (!s->any_admitted) {
-        s->any_admitted = true;
-        s->t_first_admit = r->t_admit;
-      }
-      admitted.push_back(r);
-    }
-    // a waiter may already be blocked on a run rejected here
-    if (rejected) s->cv.notify_all();
-    std::vector<Run *> act = s->active;
-    lk.unlock();
-
-    bool progressed = !admitted.empty();
-    std::vector<Run *> still;
-    std::vector<Run *> ended;
-    for (Run *r : admitted) {
-      // a run that failed to start frees its budget and wakes its waiter
-      if (start_run(r)) act.push_back(r);
-      else ended.push_back(r);
-    }
-    for (Run *r : act) {
-      if (r->state != RUN_ACTIVE) {
-        ended.push_back(r);
+// Admission (caller holds s->mu): in arrival order while memory and slots
+// allow; a run that can never fit fails at once.
+void admit_locked(abl_server *s) {
+  while (!s->queue.empty() && s->admitted < s->max_active) {
+    Run *r = s->queue.front();
+    if (s->used + r->footprint > s->budget) {
+      if (s->admitted == 0 && r->footprint > s->budget) {
+        s->queue.pop_front();
+        s->used += r->footprint;  // end_run_locked releases it
+        s->admitted += 1;
+        end_run_locked(s, r, RUN_FAILED, "run needs more device memory than the server's budget");
         continue;
       }
+      break;
+    }
+    s->queue.pop_front();
+    s->used += r->footprint;
+    s->peak = std::max(s->peak, s->used);
+    s->admitted += 1;
+    r->state = RUN_STARTING;
+    r->t_admit = Clock::now();
+    if (!s->any_admitted) {
+      s->any_admitted = true;
+      s->t_first_admit = r->t_admit;
+    }
+    s->to_start.push_back(r);
+    s->work_cv.notify_all();
+  }
+}
+
+void starter_loop(abl_server *s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->work_cv.wait(lk, [&] { return s->stop || !s->to_start.empty(); });
+    if (s->stop) return;
+    Run *r = s->to_start.front();
+    s->to_start.pop_front();
+    lk.unlock();
+    std::string err;
+    const bool ok = start_run(r, &err);
+    lk.lock();
+    if (ok) {
+      r->state = RUN_ACTIVE;
+      s->ready.push_back(r);
+      s->work_cv.notify_all();
+    } else {
+      end_run_locked(s, r, RUN_FAILED, err);
+    }
+  }
+}
+
+void finisher_loop(abl_server *s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->work_cv.wait(lk, [&] { return s->stop || !s->to_finish.empty(); });
+    if (s->stop) return;
+    Run *r = s->to_finish.front();
+    s->to_finish.pop_front();
+    lk.unlock();
+    std::string err;
+    const bool ok = finish_run(r, &err);
+    lk.lock();
+    end_run_locked(s, r, ok ? RUN_DONE : RUN_FAILED, err);
+  }
+}
+
+// The issuer: admission, then a pass over the started runs: each run whose
+// stream is idle gets its next chunk of timesteps, or goes to the finisher.
+void issuer_loop(abl_server *s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop) {
+    admit_locked(s);
+    std::vector<Run *> mine;
+    mine.swap(s->ready);
+    lk.unlock();
+
+    bool progressed = false;
+    std::vector<Run *> keep, finished;
+    std::vector<std::pair<Run *, std::string>> failed;
+    for (Run *r : mine) {
       int done = 0;
       if (abl_hip_query(r->sim, &done)) {
-        fail_run(r, "query");
-        ended.push_back(r);
+        failed.push_back({r, drop_sim(r, "query")});
         continue;
       }
       if (!done) {
-        still.push_back(r);
+        keep.push_back(r);
         continue;
       }
       if (r->issued >= r->timesteps) {
-        finish_run(r);
-        ended.push_back(r);
-        progressed = true;
+        finished.push_back(r);
         continue;
       }
       const int k = std::min(s->chunk, r->timesteps - r->issued);
       if (abl_hip_simulate(r->sim, k)) {
-        fail_run(r, "simulate");
-        ended.push_back(r);
+        failed.push_back({r, drop_sim(r, "simulate")});
         continue;
       }
       r->issued += k;
       progressed = true;
-      still.push_back(r);
+      keep.push_back(r);
     }
 
     lk.lock();
-    s->active = still;
-    for (Run *r : ended) {
-      s->used -= r->footprint;
-      if (r->state == RUN_DONE) {
-        s->agent_steps += (double) r->agents * (double) r->timesteps;
-        s->t_last_done = r->t_done;
-      }
+    s->ready.insert(s->ready.end(), keep.begin(), keep.end());
+    for (Run *r : finished) {
+      r->state = RUN_FINISHING;
+      s->to_finish.push_back(r);
+      progressed = true;
     }
-    if (!ended.empty()) s->cv.notify_all();
-    if (!progressed) s->cv.wait_for(lk, std::chrono::microseconds(200));
+    if (!finished.empty()) s->work_cv.notify_all();
+    for (auto &f : failed) end_run_locked(s, f.first, RUN_FAILED, f.second);
+    if (!progressed) s->work_cv.wait_for(lk, std::chrono::microseconds(200));
   }
 }
 
 double seconds(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double>(b - a).count();
+}
+
+int public_state(int st) {
+  return st == RUN_STARTING || st == RUN_FINISHING ? RUN_ACTIVE : st;
 }
 
 }  // namespace
@@ -253,13 +325,17 @@ extern "C" int abl_server_create(size_t hbm_budget, int max_active, int chunk_st
   s->budget = hbm_budget;
   s->max_active = max_active;
   s->chunk = chunk_steps;
-  s->worker = std::thread(worker_loop, s);
+  s->issuer = std::thread(issuer_loop, s);
+  s->starter = std::thread(starter_loop, s);
+  s->finisher = std::thread(finisher_loop, s);
   *out = s;
   return ABL_OK;
 }
 
 extern "C" int abl_server_submit(abl_server *s, const abl_run_request *req, uint64_t *run_id) {
-  if (!s || !req || !req->program || req->timesteps < 0 || (req->num_params && !req->param_names))
+  if (!s || !req || !req->program || req->timesteps < 0 || req->num_params < 0 ||
+      (req->num_params && (!req->param_names || !req->param_values)) || req->num_config < 0 ||
+      (req->num_config && (!req->config_keys || !req->config_values)))
     return serve_fail(ABL_ERR_ARG, "bad request");
   Run *r = new Run();
   r->program = req->program;
@@ -268,12 +344,19 @@ extern "C" int abl_server_submit(abl_server *s, const abl_run_request *req, uint
     r->names.push_back(req->param_names[k]);
     r->values.push_back(req->param_values[k]);
   }
+  for (int k = 0; k < req->num_config; k++) {
+    if (!req->config_keys[k] || !req->config_values[k]) {
+      delete r;
+      return serve_fail(ABL_ERR_ARG, "bad request");
+    }
+    r->config.push_back({req->config_keys[k], req->config_values[k]});
+  }
   r->timesteps = req->timesteps;
   r->rng[0] = req->rng_state[0];
   r->rng[1] = req->rng_state[1];
   size_t fp = 0;
-  if (abl_hip_estimate_footprint(req->program, req->precision, req->agents_hint, req->cells_hint,
-                                 &fp)) {
+  if (abl_hip_estimate_footprint(req->program, req->precision, req->num_params, req->param_names,
+                                 req->param_values, req->agents_hint, req->cells_hint, &fp)) {
     std::string m = abl_hip_last_error();
     delete r;
     return serve_fail(ABL_ERR_ARG, m);
@@ -284,7 +367,7 @@ extern "C" int abl_server_submit(abl_server *s, const abl_run_request *req, uint
   r->id = s->next_id++;
   s->runs[r->id] = r;
   s->queue.push_back(r);
-  s->cv.notify_all();
+  s->work_cv.notify_all();
   if (run_id) *run_id = r->id;
   return ABL_OK;
 }
@@ -299,7 +382,7 @@ extern "C" int abl_server_status(abl_server *s, uint64_t id, int *state) {
   std::lock_guard<std::mutex> g(s->mu);
   Run *r = find_run(s, id);
   if (!r) return serve_fail(ABL_ERR_ARG, "unknown run");
-  *state = r->state;
+  *state = public_state(r->state);
   return ABL_OK;
 }
 
@@ -389,8 +472,11 @@ extern "C" int abl_server_destroy(abl_server *s) {
     std::lock_guard<std::mutex> g(s->mu);
     s->stop = true;
     s->cv.notify_all();
+    s->work_cv.notify_all();
   }
-  s->worker.join();
+  s->issuer.join();
+  s->starter.join();
+  s->finisher.join();
   for (auto &kv : s->runs) {
     if (kv.second->sim) abl_hip_destroy(kv.second->sim);
     delete kv.second;

@@ -1008,27 +1008,46 @@ template <class M>
 using policy_pos_t = typename std::conditional<policy_relative<M>::value, double, typename M::F>::type;
 
 // The gated re-sort and list build of one timestep in one launch (256
-// threads, all blocks resident): bin, scan, place (abl_pipeline.hpp,
-// delta_resort), then every tile's lists.  Returns at once when the previous
-// step set no gate, or when an earlier launch abandoned a barrier (the host
-// then sees bar->error and rebuilds from scratch).
+// threads): bin, scan, place (abl_pipeline.hpp), then the copy of the new
+// cell offsets and every tile's lists, as phases of a work queue whose waits
+// end whether or not the blocks are co-resident (abl_pipeline.hpp).  Returns
+// at once when the previous step set no gate, or after a wait ran past its
+// spin limit (the host then sees q->error and rebuilds from scratch).
 template <class M, int TX, int TY, int TZ, int RM>
 __global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListArgs nl,
                                                       DeltaArgs d) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (nl.gate[nl.parity] == 0u) return;
-  if (__hip_atomic_load(&d.bar->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  if (__hip_atomic_load(&d.q->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
   __shared__ uint32_t lds[kWave + 1];
-  if (!delta_resort<policy_pos_t<M>>(d, lds)) return;
-  list_copy_starts(nl);
   const unsigned nt = tile_count<TX, TY, TZ, M::DIM>(a.g);
+  const uint32_t chunks = (uint32_t) (d.n + 255) / 256u;
+  const uint32_t ncopy = nl.start_dst ? (uint32_t) (nl.ncells + 1 + 255) / 256u : 0u;
+  QueuePlan plan;
+  plan.end[0] = chunks;
+  plan.end[1] = plan.end[0] + (uint32_t) d.nscan;
+  plan.end[2] = plan.end[1] + chunks;
+  plan.end[3] = plan.end[2] + ncopy + nt;
   uint32_t longest = 0;
-  for (unsigned t = blockIdx.x; t < nt; t += gridDim.x) {
-    longest = max(longest, list_build_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(t, nt), smem));
-    __syncthreads();  // every wave is done with the tile's LDS
+  for (;;) {
+    const uint32_t item = queue_claim(d.q);
+    if (item >= plan.end[3]) break;
+    const int phase = plan.phase_of(item);
+    if (phase > 0 && !queue_wait(d.q, phase - 1, plan.items(phase - 1), d.spins)) break;
+    const uint32_t k = item - plan.first(phase);
+    if (phase < 3) {
+      delta_item<policy_pos_t<M>>(d, phase, k, lds);
+    } else if (k < ncopy) {
+      const int c = (int) (k * 256u + threadIdx.x);
+      if (c <= nl.ncells) nl.start_dst[c] = nl.start_src[c];
+    } else {
+      const unsigned t = k - ncopy;
+      longest = max(longest, list_build_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(t, nt), smem));
+    }
+    queue_finish(d.q, phase);
   }
   list_longest(nl, longest);
-  grid_barrier_exit(d.bar);
+  queue_exit(d.q);
 }
 
 // One timestep of the step through the lists.  An agent whose list
@@ -1192,8 +1211,9 @@ inline void launch_list(typename M::Args a, const GridView &g, float occupancy, 
   const int threads = block > 256 ? 256 : block;
   const unsigned ntiles = tile_count<TX, TY, TZ, M::DIM>(g);
   if (mode == 3) {
-    // half of what the CUs hold at once, a multiple of the 8 XCDs: every
-    // block is resident, so the grid barriers complete
+    // as many blocks as the CUs hold at once (a multiple of the 8 XCDs); the
+    // work queue does not need them resident together, more would only
+    // find the queue empty
     if (!delta || delta->pos_elem != (int) sizeof(policy_pos_t<M>) || !delta->launched) return;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
@@ -1210,10 +1230,11 @@ inline void launch_list(typename M::Args a, const GridView &g, float occupancy, 
               smem) != hipSuccess ||
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return;
-      grid = std::min(((unsigned) per_cu * (unsigned) cus / 2u) & ~7u, 0xFFF8u);
+      grid = std::min(((unsigned) per_cu * (unsigned) cus) & ~7u, 0xFFF8u);
       if (grid < 8u) return;
       cached.store(key | grid, std::memory_order_relaxed);
     }
+    if (delta->blocks) grid = delta->blocks;
     hipLaunchKernelGGL((tiled_rebuild<M, TX, TY, TZ, RM>), grid, 256, smem, stream, a, nl, *delta);
     *delta->launched = 1;
     return;

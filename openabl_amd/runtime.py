@@ -64,7 +64,8 @@ _SIGNATURES = {
     "abl_hip_log_csv": ([C.c_void_p, C.c_char_p], C.c_int),
     "abl_hip_log_text": ([C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p], C.c_int),
     "abl_hip_query": ([C.c_void_p, C.c_void_p], C.c_int),
-    "abl_hip_estimate_footprint": ([C.c_char_p, C.c_int, C.c_size_t, C.c_size_t, C.c_void_p], C.c_int),
+    "abl_hip_estimate_footprint": ([C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_size_t, C.c_size_t, C.c_void_p], C.c_int),
     "abl_server_last_error": ([], C.c_char_p),
     "abl_server_create": ([C.c_size_t, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "abl_server_submit": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),

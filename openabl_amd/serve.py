@@ -21,7 +21,9 @@ class RunRequest(C.Structure):
     _fields_ = [("program", C.c_char_p), ("precision", C.c_int), ("num_params", C.c_int),
                 ("param_names", C.POINTER(C.c_char_p)), ("param_values", C.POINTER(C.c_double)),
                 ("timesteps", C.c_int), ("rng_state", C.c_uint64 * 2),
-                ("agents_hint", C.c_size_t), ("cells_hint", C.c_size_t)]
+                ("agents_hint", C.c_size_t), ("cells_hint", C.c_size_t),
+                ("num_config", C.c_int), ("config_keys", C.POINTER(C.c_char_p)),
+                ("config_values", C.POINTER(C.c_char_p))]
 
 
 def _check(rc: int):
@@ -54,6 +56,25 @@ def sizing(program: str, params: dict) -> tuple[int, int]:
     return n, int(cells) * 4  # up to hip.cell_scale 2 in 2-D
 
 
+def estimate_footprint(program: str, precision: str, params: dict, agents: int,
+                       cells: int = 0) -> int:
+    """Upper bound of the device bytes a run of `program` with these
+    parameters holds while it has at most `agents` agents per type
+    (abl_hip_estimate_footprint)."""
+    p = derived_params(program, params or {})
+    names = [k.encode() for k in p]
+    arr_n = (C.c_char_p * max(len(names), 1))(*names)
+    arr_v = (C.c_double * max(len(p), 1))(*[float(v) for v in p.values()])
+    out = C.c_size_t()
+    lib = load_library()
+    rc = lib.abl_hip_estimate_footprint(program.encode(), {"f64": 0, "f32": 1, "f32c": 2}[precision],
+                                        len(names), arr_n, arr_v, int(agents), int(cells),
+                                        C.byref(out))
+    if rc != 0:
+        raise AblError(rc, lib.abl_hip_last_error().decode())
+    return out.value
+
+
 class Server:
     def __init__(self, hbm_budget: int, max_active: int = 8, chunk_steps: int = 20):
         self.lib = load_library()
@@ -63,14 +84,18 @@ class Server:
         self._keep = {}
 
     def submit(self, program: str, precision: str = "f64", params: dict | None = None,
-               timesteps: int = 100, rng_state=REF_RNG) -> int:
+               timesteps: int = 100, rng_state=REF_RNG, config: dict | None = None) -> int:
         p = derived_params(program, params or {})
         names = [k.encode() for k in p]
         arr_n = (C.c_char_p * max(len(names), 1))(*names)
         arr_v = (C.c_double * max(len(p), 1))(*[float(v) for v in p.values()])
+        cfg = config or {}
+        arr_k = (C.c_char_p * max(len(cfg), 1))(*[k.encode() for k in cfg])
+        arr_c = (C.c_char_p * max(len(cfg), 1))(*[str(v).encode() for v in cfg.values()])
         n, cells = sizing(program, params or {})
-        req = RunRequest(program.encode(), {"f64": 0, "f32": 1}[precision], len(names), arr_n,
-                         arr_v, int(timesteps), (C.c_uint64 * 2)(*rng_state), n, cells)
+        req = RunRequest(program.encode(), {"f64": 0, "f32": 1, "f32c": 2}[precision], len(names),
+                         arr_n, arr_v, int(timesteps), (C.c_uint64 * 2)(*rng_state), n, cells,
+                         len(cfg), arr_k, arr_c)
         rid = C.c_uint64()
         _check(self.lib.abl_server_submit(self.h, C.byref(req), C.byref(rid)))
         self._keep[rid.value] = (program, precision)

@@ -8,9 +8,11 @@ than skin / 2; rebuilds are decided on the device.
     walk (hip.nlist=0);
   * lists too short to hold an agent's candidates (hip.list_cap) send it
     back to the stencil walk on a fresh grid: again bit-identical;
-  * the gated re-sort and list build run as one launch with grid barriers
-    (hip.rebuild=fused, the default) or as four launches (split): the two
-    are bit-identical over runs that reuse and rebuild the lists;
+  * the gated re-sort and list build run as one launch whose phases a work
+    queue orders (hip.rebuild=fused, the default) or as four launches
+    (split): the two are bit-identical over runs that reuse and rebuild the
+    lists, also when the launch has many times more blocks than the device
+    holds at once and when several simulations rebuild at the same time;
   * with the default skin the lists are reused for many steps: the same
     neighbours are found (exact near test on every list entry), visited in
     the order of the last build, so results match the stencil walk to the
@@ -122,11 +124,30 @@ def test_one_launch_rebuild_equals_split(model, prec, params, cfg):
             assert field_bytes(x) == field_bytes(y), f"{model} {prec} skin {skin}: step {t} differs"
 
 
-def test_abandoned_one_launch_rebuild_recovers():
-    """A one-launch rebuild whose barrier waits run out (forced here with a
-    spin limit of 0, so every block leaves at once) reports ABL_ERR_DEVICE at
-    the next synchronisation; the runtime then rebuilds from scratch with the
-    split launches and the run continues."""
+def test_one_launch_rebuild_oversubscribed_equals_split():
+    """The one-launch rebuild's phase waits do not need its blocks resident
+    together: with 65,528 blocks (far more than the CUs hold at once, so most
+    start only after others have left) the run at the benchmark's launch
+    configuration, over steps that reuse and rebuild the lists, is
+    bit-identical to the four-launch rebuild."""
+    print("case nlist oversubscribed one-launch rebuild", flush=True)
+    params = boids2d_params(num_agents=65536)
+    cfg = {"hip.cell_scale": 2, "hip.lds_slack": 1.25}
+    steps = (1, 4, 15)
+    for skin in (1e-9, 0.05):
+        a = run("boids2d", "f64", params, steps,
+                **dict(cfg, **{"hip.skin": skin, "hip.rebuild_blocks": 65528}))
+        b = run("boids2d", "f64", params, steps, **dict(cfg, **{"hip.skin": skin, "hip.rebuild": "split"}))
+        for t, x, y in zip(steps, a, b):
+            assert field_bytes(x) == field_bytes(y), f"skin {skin}: step {t} differs"
+
+
+def test_forced_spin_limit_is_reported_and_recovers():
+    """A phase wait that runs past its spin limit -- impossible at the default
+    limit, forced here with a limit of 0 so the first wait gives up --
+    abandons the rebuild: the next synchronisation reports ABL_ERR_DEVICE, the
+    runtime rebuilds from scratch with the split launches and later runs are
+    correct again (equal to the oracle from the state they start from)."""
     from openabl_amd import MEMBERS, AblError, Simulation, load_library, record_dtype
     print("case nlist abandoned one-launch rebuild", flush=True)
     load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
@@ -137,11 +158,19 @@ def test_abandoned_one_launch_rebuild_recovers():
     sim.simulate(3)
     with pytest.raises(AblError):
         sim.synchronize()
-    sim.simulate(3)
+    dt = record_dtype(MEMBERS["boids2d"], "f64")
+    s0 = sim.download(dt)
+    sim.simulate(1)
     sim.synchronize()
-    out = sim.download(record_dtype(MEMBERS["boids2d"], "f64"))
+    out = sim.download(dt)
     sim.close()
-    assert len(out) == 4096 and np.isfinite(out["pos"]).all()
+    o = Oracle("boids2d").set(**params)
+    o.set_agents(s0)
+    o.simulate(1)
+    want = o.agents()
+    for f in ("pos", "velocity"):
+        tol = 1e-12 * np.maximum(np.abs(want[f]), 1.0)
+        assert (np.abs(out[f] - want[f]) <= tol).all(), f
 
 
 def test_tile_beyond_16bit_indices_walks_stencil():
@@ -161,9 +190,9 @@ def test_tile_beyond_16bit_indices_walks_stencil():
 
 def test_two_live_simulations_equal_alone():
     """Two live handles stepping in turn (each on its own stream, so their
-    launches overlap on the device) rebuild their lists with the four-launch
-    form; every timestep still rebuilds (vanishing skin) and both runs equal
-    the same runs made alone, bit for bit."""
+    one-launch rebuilds overlap on the device); every timestep rebuilds
+    (vanishing skin) and both runs equal the same runs made alone, bit for
+    bit."""
     from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
     print("case nlist two live simulations", flush=True)
     cfg = {"hip.cell_scale": 2, "hip.skin": 1e-9}

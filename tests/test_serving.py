@@ -61,13 +61,11 @@ def test_mix_results_equal_alone_runs():
 def test_admission_respects_budget():
     """A budget that holds about two of these runs at once: all complete,
     the peak admitted footprint never exceeds the budget."""
-    from openabl_amd.serve import Server, sizing
-    from openabl_amd import load_library
-    n, cells = sizing("boids2d", {"num_agents": 200000, "agent_density": 0.065536})
-    import ctypes as C
-    one = C.c_size_t()
-    load_library().abl_hip_estimate_footprint(b"boids2d", 0, n, cells, C.byref(one))
-    budget = int(2.5 * one.value)
+    from openabl_amd.serve import Server, estimate_footprint, sizing
+    params = {"num_agents": 200000, "agent_density": 0.065536}
+    n, cells = sizing("boids2d", params)
+    one = estimate_footprint("boids2d", "f64", params, n, cells)
+    budget = int(2.5 * one)
     srv = Server(hbm_budget=budget, max_active=8, chunk_steps=10)
     rids = [srv.submit("boids2d", "f64", {"num_agents": 200000, "agent_density": 0.065536}, 40)
             for _ in range(5)]
@@ -87,4 +85,79 @@ def test_oversized_run_fails_cleanly():
     with pytest.raises(AblError):
         srv.wait(rid)
     assert srv.status(rid) == 3
+    srv.close()
+
+
+FOOTPRINT_CASES = [
+    ("boids2d", "f64", {"num_agents": 1 << 20, "agent_density": 0.065536}, 30),
+    ("boids2d", "f32", {"num_agents": 65536, "agent_density": 0.065536}, 30),
+    ("circle", "f64", {"num_agents": 100000, "rho": 0.1024, "r": 2.5}, 10),
+    ("circle3d", "f64", {"num_agents": 262144, "rho": 262144 / 1e9, "r": 4.0,
+                         "num_clusters": 16, "cluster_sigma": 20.0}, 3),
+    ("game_of_life", "f64", {"num_agents": 1024 * 1024, "alive_fraction": 0.3}, 5),
+    ("predator_prey", "f64", {"num_agents": 100000, "prey_fraction": 0.4,
+                              "predator_fraction": 0.04}, 20),
+]
+
+
+@pytest.mark.parametrize("prog,prec,params,steps", FOOTPRINT_CASES,
+                         ids=[f"{c[0]}-{c[1]}" for c in FOOTPRINT_CASES])
+def test_estimate_bounds_live_footprint(prog, prec, params, steps):
+    """The admission estimate is an upper bound of what a live simulation
+    holds after its grid, neighbour lists and (for predator_prey) flag and
+    newborn buffers exist: abl_hip_footprint <= abl_hip_estimate_footprint."""
+    from openabl_amd import Simulation, load_library
+    from openabl_amd.serve import estimate_footprint, sizing
+    print(f"case footprint {prog} {prec}", flush=True)
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    sim = Simulation(prog, prec, params=params)
+    sim.init_agents()
+    sim.simulate(steps)
+    sim.synchronize()
+    live = sim.footprint()
+    sim.close()
+    n, cells = sizing(prog, params)
+    est = estimate_footprint(prog, prec, params, n, cells)
+    print(f"  live {live} estimate {est} ({est / live:.2f}x)", flush=True)
+    assert live <= est
+
+
+def test_status_polling_while_runs_finish():
+    """One thread polls status, run times and stats while runs start and
+    finish on the server's threads: every run seen DONE already has its end
+    time and results (latencies non-negative, admit <= done), and the final
+    statistics agree with the runs' own times."""
+    import threading
+    from openabl_amd.serve import Server
+    srv = Server(hbm_budget=8 << 30, max_active=3, chunk_steps=5)
+    rids = [srv.submit("circle", "f64", {"num_agents": 2000 + 500 * k, "rho": 0.1024, "r": 2.5},
+                       20 + 10 * k, (0xDEADBEEF + k, 0xBEEFDEAD)) for k in range(8)]
+    seen, bad = set(), []
+    stop = threading.Event()
+
+    def poll():
+        while not stop.is_set():
+            for rid in rids:
+                if srv.status(rid) == 2:
+                    t = srv.times(rid)
+                    if not (t[0] >= 0 and t[1] >= 0 and abs(t[2] - t[0] - t[1]) < 1e-6):
+                        bad.append((rid, t))
+                    if len(srv.result(rid)) == 0:
+                        bad.append((rid, "no result"))
+                    seen.add(rid)
+            st = srv.stats()
+            if st["completed"] and not (st["p50_s"] > 0 and st["p99_s"] >= st["p50_s"]):
+                bad.append(("stats", st))
+
+    th = threading.Thread(target=poll)
+    th.start()
+    for rid in rids:
+        srv.wait(rid)
+    stop.set()
+    th.join()
+    assert not bad, bad[:3]
+    st = srv.stats()
+    lat = sorted(srv.times(r)[2] for r in rids)
+    assert st["completed"] == len(rids)
+    assert abs(st["p50_s"] - lat[3]) < 1e-9 and abs(st["p99_s"] - lat[-1]) < 1e-9
     srv.close()
