@@ -218,6 +218,18 @@ int abl_hip_download_ids(abl_sim *sim, int agent_type, int64_t *ids, size_t capa
 int abl_hip_reduce_sum(abl_sim *sim, int agent_type, const char *member, double *out);
 int abl_hip_reduce_count_eq(abl_sim *sim, int agent_type, const char *member, double value,
                             size_t *count);
+
+/* sum(Type.member) exactly as abl_hip_reduce_sum computes it (same kernel,
+ * same partial sums, same result bits), but allocating nothing: no
+ * hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The partial sums live in a static 8 KiB device array, one per GPU (at
+ * most 1024 blocks, each striding over the column, so any agent count
+ * fits); the array's device address is looked up on the first call on each
+ * device.  Calls on one GPU share the array and are serialised by a host
+ * lock; a call waits for the sim's stream. */
+int abl_hip_reduce_sum_noalloc(abl_sim *sim, int agent_type, const char *member, double *out);
 int abl_hip_last_exec_time(abl_sim *sim, double *seconds);
 int abl_hip_log_csv(abl_sim *sim, const char *row);
 int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len);

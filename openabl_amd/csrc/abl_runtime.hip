@@ -1831,6 +1831,94 @@ extern "C" int abl_hip_reduce_sum(abl_sim *sim, int t, const char *member, doubl
   return ABL_OK;
 }
 
+// ---- sum(Type.member) without allocation ----------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops that forbid it): the per-block partial sums live in
+// a file-scope device array, one copy per GPU, whose address is looked up
+// once per device.  Host code never passes the array's name to a kernel or
+// copy, only that address.  The kernel writes every partial it launches
+// before the host reads them back, so nothing needs clearing between calls;
+// calls on one device are serialised by a host lock, since they share the
+// array.
+
+constexpr int kNoAllocPartials = 1024;  // blocks of the reduction, at most
+static __device__ __align__(256) double scratch[kNoAllocPartials];  // 8 KiB per GPU
+constexpr int kMaxDevices = 64;
+
+static std::mutex g_scratch_mu[kMaxDevices];
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `dev` (looked up once, with dev current).
+static int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(ABL_ERR_UNSUPPORTED, "device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    const hipError_t e2 = hipSetDevice(prev);
+    if (e != hipSuccess)
+      return fail(ABL_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(ABL_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e2));
+    if (!p) return fail(ABL_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ABL_OK;
+}
+
+extern "C" int abl_hip_reduce_sum_noalloc(abl_sim *sim, int t, const char *member, double *out) {
+  CHECK_SIM(sim);
+  int c0 = 0, nc = 0;
+  int rc = member_columns(sim, t, member, &c0, &nc);
+  if (rc) return rc;
+  if (!out) return fail(ABL_ERR_ARG, "null output");
+  AgentStore &s = sim->types[t];
+  const int n = (int) s.n;
+  // every agent count fits: the kernel strides over the column with at most
+  // kNoAllocPartials blocks, one partial each
+  const int blocks = std::max(1, std::min(kNoAllocPartials, (n + 255) / 256));
+  if (blocks > kNoAllocPartials) return fail(ABL_ERR_ARG, "reduction exceeds the scratch array");
+  const int kind = s.desc->members[s.cols[c0].member].kind;
+  std::lock_guard<std::mutex> lock(g_scratch_mu[sim->device < 0 || sim->device >= kMaxDevices ? 0 : sim->device]);
+  void *ptr = nullptr;
+  rc = scratch_address(sim->device, &ptr);
+  if (rc) return rc;
+  double partial[kNoAllocPartials];  // host copy of the partials (stack, no allocation)
+  for (int k = 0; k < nc; k++) {
+    const void *col = col_cur(s, c0 + k);
+    if (n == 0) {
+      out[k] = 0.0;
+      continue;
+    }
+    if (kind == K_BOOL) {
+      hipLaunchKernelGGL((k_reduce_sum<uint8_t, long long>), blocks, 256, 0, sim->stream,
+                         (const uint8_t *) col, n, (long long *) ptr);
+    } else if (kind == K_INT) {
+      hipLaunchKernelGGL((k_reduce_sum<int32_t, long long>), blocks, 256, 0, sim->stream,
+                         (const int32_t *) col, n, (long long *) ptr);
+    } else if (s.cols[c0].elem == 8) {
+      hipLaunchKernelGGL((k_reduce_sum<double, double>), blocks, 256, 0, sim->stream,
+                         (const double *) col, n, (double *) ptr);
+    } else {
+      hipLaunchKernelGGL((k_reduce_sum<float, double>), blocks, 256, 0, sim->stream,
+                         (const float *) col, n, (double *) ptr);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(partial, ptr, (size_t) blocks * 8, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    double acc = 0.0;
+    long long iacc = 0;
+    for (int b = 0; b < blocks; b++) {
+      if (kind == K_BOOL || kind == K_INT) iacc += ((const long long *) partial)[b];
+      else acc += partial[b];
+    }
+    out[k] = (kind == K_BOOL || kind == K_INT) ? (double) iacc : acc;
+  }
+  return ABL_OK;
+}
+
 // count(Type.member, value): agents whose (scalar) member equals value.
 extern "C" int abl_hip_reduce_count_eq(abl_sim *sim, int t, const char *member, double value,
                                        size_t *out) {

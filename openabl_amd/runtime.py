@@ -59,6 +59,7 @@ _SIGNATURES = {
     "abl_hip_last_timing": ([C.c_void_p, C.c_void_p], C.c_int),
     "abl_hip_download_ids": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p], C.c_int),
     "abl_hip_reduce_sum": ([C.c_void_p, C.c_int, C.c_char_p, C.c_void_p], C.c_int),
+    "abl_hip_reduce_sum_noalloc": ([C.c_void_p, C.c_int, C.c_char_p, C.c_void_p], C.c_int),
     "abl_hip_reduce_count_eq": ([C.c_void_p, C.c_int, C.c_char_p, C.c_double, C.c_void_p], C.c_int),
     "abl_hip_last_exec_time": ([C.c_void_p, C.c_void_p], C.c_int),
     "abl_hip_log_csv": ([C.c_void_p, C.c_char_p], C.c_int),
