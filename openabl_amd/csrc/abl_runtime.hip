@@ -35,6 +35,7 @@
 #include "../../include/abl_hip.h"
 #include "abl_runtime.hpp"
 #include "abl_pipeline.hpp"
+#include "abl_gen_device.hpp"
 
 using namespace abl;
 
@@ -87,8 +88,30 @@ static const ProgramDesc *const *programs() {
   return table;
 }
 
+// Programs registered at load time by generated step libraries
+// (register_program), after the built-in ones; a registered program of an
+// existing name replaces it for later lookups.
+static std::mutex g_registry_mu;
+static std::vector<const ProgramDesc *> &registered() {
+  static std::vector<const ProgramDesc *> v;
+  return v;
+}
+
+bool abl::register_program(const ProgramDesc *desc) {
+  if (!desc || !desc->name) return false;
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  registered().push_back(desc);
+  return true;
+}
+
 const ProgramDesc *abl::find_program(const char *name) {
-  for (int i = 0; name && programs()[i]; i++) {
+  if (!name) return nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_registry_mu);
+    for (auto it = registered().rbegin(); it != registered().rend(); ++it)
+      if (std::strcmp((*it)->name, name) == 0) return *it;
+  }
+  for (int i = 0; programs()[i]; i++) {
     if (std::strcmp(programs()[i]->name, name) == 0) return programs()[i];
   }
   return nullptr;
@@ -97,12 +120,17 @@ const ProgramDesc *abl::find_program(const char *name) {
 extern "C" int abl_hip_program_count(void) {
   int n = 0;
   while (programs()[n]) n++;
-  return n;
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  return n + (int) registered().size();
 }
 
 extern "C" const char *abl_hip_program_name(int index) {
-  if (index < 0 || index >= abl_hip_program_count()) return nullptr;
-  return programs()[index]->name;
+  int n = 0;
+  while (programs()[n]) n++;
+  if (index >= 0 && index < n) return programs()[index]->name;
+  std::lock_guard<std::mutex> g(g_registry_mu);
+  if (index >= n && index < n + (int) registered().size()) return registered()[index - n]->name;
+  return nullptr;
 }
 
 // ---- storage ---------------------------------------------------------------
@@ -766,6 +794,7 @@ static int reach_for(const AgentStore &s, double radius) {
 }
 
 static double step_radius(const abl_sim *sim, const StepDesc &st) {
+  if (st.radius_fn) return st.radius_fn(sim->precision);
   for (size_t p = 0; p < sim->prog->params.size(); p++) {
     if (std::strcmp(sim->prog->params[p].name, st.radius_param) == 0) {
       // under use_float the parameter is a float global, widened for the
@@ -1564,12 +1593,13 @@ extern "C" int abl_hip_simulate(abl_sim *sim, int timesteps) {
       int rc = launch_step(sim, st, ordinal++);
       if (rc) return rc;
     }
-    if (sim->prog->seq_step) {
+    if (sim->prog->seq_step || sim->prog->seq_step_prec) {
       // getLastExecTime(): seconds taken by this timestep's parallel steps
       HIP_TRY(hipStreamSynchronize(sim->stream));
       auto now = std::chrono::steady_clock::now();
       sim->last_exec = std::chrono::duration<double>(now - t_prev).count();
-      int rc = sim->prog->seq_step(sim, sim->params.data());
+      int rc = sim->prog->seq_step_prec ? sim->prog->seq_step_prec(sim, sim->precision)
+                                        : sim->prog->seq_step(sim, sim->params.data());
       if (rc) return rc;
       t_prev = std::chrono::steady_clock::now();
     }
@@ -1652,8 +1682,8 @@ extern "C" int abl_hip_estimate_footprint(const char *program, int precision, in
     double list_entries = 0.0;
     for (const StepDesc &st : prog->steps) {
       if (st.near_agent != (int) t || s.dim == 0 || !(gran > 0.0)) continue;
-      double r = 0.0;
-      for (size_t p = 0; p < prog->params.size(); p++)
+      double r = st.radius_fn ? st.radius_fn(precision) : 0.0;
+      for (size_t p = 0; !st.radius_fn && p < prog->params.size(); p++)
         if (std::strcmp(prog->params[p].name, st.radius_param) == 0) r = st.radius_scale * params[p];
       const double cell = gran / 4.0;  // hip.cell_scale at most 4
       double nc = 1.0, vol = 1.0;

@@ -76,6 +76,10 @@ struct StepDesc {
   bool removes;              // calls removeCurrent()
   int adds;                  // 0, or 1 + the agent type it add()s at run time
   bool nlist;                // the launcher supports neighbour lists (StepLaunch::nl_mode)
+  // near radius as the program computes it at `precision` (generated
+  // programs, whose constants are folded literals); when set it replaces
+  // radius_scale * radius_param
+  double (*radius_fn)(int precision);
 };
 
 struct ProgramDesc {
@@ -96,6 +100,9 @@ struct ProgramDesc {
   const EnsembleHooks *ensemble;
   // has an ABL_PREC_F32C form (FP32 neighbour arithmetic, FP64 positions)
   bool f32c;
+  // sequential step taking the sim's precision (generated programs), used
+  // instead of seq_step when set
+  int (*seq_step_prec)(struct ::abl_sim *sim, int precision);
 };
 
 // Column pointers of one agent type (members in declaration order, vector

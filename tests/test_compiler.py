@@ -1,0 +1,227 @@
+"""The .abl compiler with the -b hip backend (openabl_amd/bin/openabl-hip), on
+the CPU: front-end diagnostics against the reference's lint corpus, the
+command-line contract, overload naming, constant folding and the shape of
+the generated programs.
+
+The lint corpus (test/*.abl with the expected stderr in test/*.exp) and the
+example models are read from the reference checkout when it is present; the
+tests that need them are skipped elsewhere.  Expected texts are compared as
+the reference's test.sh compares them (diff -b: runs of blanks equal).
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ABLC = os.path.join(REPO, "openabl_amd", "bin", "openabl-hip")
+REF = "/root/reference"
+REF_TESTS = os.path.join(REF, "test")
+REF_EXAMPLES = os.path.join(REF, "examples")
+OWN_EXAMPLES = os.path.join(REPO, "examples")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(ABLC), reason="openabl-hip not built")
+
+
+def ablc(*args, cwd=None):
+    return subprocess.run([ABLC, *args], capture_output=True, text=True, cwd=cwd)
+
+
+def squash(text):
+    return [re.sub(r"\s+", " ", line).strip() for line in text.strip().splitlines()]
+
+
+def lint_cases():
+    if not os.path.isdir(REF_TESTS):
+        return []
+    return sorted(f[:-4] for f in os.listdir(REF_TESTS) if f.endswith(".abl"))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_TESTS), reason="reference lint corpus not present")
+@pytest.mark.parametrize("case", lint_cases())
+def test_lint_corpus(case):
+    r = ablc("--lint-only", "-i", os.path.join(REF_TESTS, case + ".abl"))
+    with open(os.path.join(REF_TESTS, case + ".exp")) as f:
+        want = f.read()
+    assert squash(r.stderr) == squash(want)
+    assert r.returncode == (1 if want.strip() else 0)
+
+
+def example_models():
+    out = []
+    for d in (REF_EXAMPLES, OWN_EXAMPLES):
+        if os.path.isdir(d):
+            out += [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".abl")]
+    return out
+
+
+@pytest.mark.parametrize("path", example_models(), ids=os.path.basename)
+def test_examples_lint_clean(path):
+    r = ablc("--lint-only", "-i", path)
+    assert r.returncode == 0 and r.stderr == ""
+
+
+def model(tmp_path, text, name="m"):
+    p = tmp_path / f"{name}.abl"
+    p.write_text(text)
+    return str(p)
+
+
+CIRCLE_MIN = """
+agent Point { position float2 pos; }
+param int num_agents = 1000;
+param float rho = 0.05;
+param float r = 5;
+float W = sqrt(num_agents / rho);
+environment { max: float2(W) }
+step move(Point in -> out) {
+  float2 p = in.pos;
+  for (Point nx : near(in, 2*r)) { p += (nx.pos - in.pos) * 0.001; }
+  out.pos = clamp(p, float2(W));
+}
+void main() {
+  for (int i : 0..num_agents) { add(Point { pos: random(float2(W)) }); }
+  simulate(10) { move }
+  save("out.json");
+}
+"""
+
+
+def test_cli_contract(tmp_path):
+    # option errors print the help and exit 1
+    r = ablc()
+    assert r.returncode == 1 and "Missing input file (-i or --input)" in r.stderr
+    r = ablc("-i", "x.abl", "-Q")
+    assert r.returncode == 1 and "Missing argument for option \"-Q\"" in r.stderr
+    r = ablc("-i", "x.abl")
+    assert r.returncode == 1 and "Missing backend (-b or --backend)" in r.stderr
+    r = ablc("-i", "x.abl", "-P", "nonsense", "-b", "hip")
+    assert r.returncode == 1 and 'Malformed parameter: Missing "="' in r.stderr
+    assert ablc("-h").returncode == 0
+    m = model(tmp_path, CIRCLE_MIN)
+    r = ablc("-i", str(tmp_path / "missing.abl"), "-b", "hip", "-o", str(tmp_path))
+    assert r.returncode == 1 and "could not be opened" in r.stderr
+    # only the hip backend is built here
+    r = ablc("-i", m, "-b", "c", "-o", str(tmp_path / "c"))
+    assert r.returncode == 1 and 'Unknown backend "c"' in r.stderr
+    r = ablc("-i", m, "-b", "hip")
+    assert r.returncode == 1 and "Missing output directory (-o or --output-dir)" in r.stderr
+    # -P of an unknown parameter, and of a constant that is not a param
+    r = ablc("-i", m, "-b", "hip", "-o", str(tmp_path / "o"), "-P", "nope=3")
+    assert r.returncode == 1 and 'Unknown parameter "nope" specified through -P on line 1' in r.stderr
+    r = ablc("-i", m, "-b", "hip", "-o", str(tmp_path / "o"), "-P", "W=3")
+    assert r.returncode == 1 and 'Only constants marked with "param"' in r.stderr
+    # a feature the backend does not support: exit 2
+    unsupported = CIRCLE_MIN.replace("float2 p = in.pos;",
+                                     "float2 p = in.pos; for (int x : new int[3]) { p.x += 0.0; }")
+    r = ablc("-i", model(tmp_path, unsupported, "u"), "-b", "hip", "-o", str(tmp_path / "u"))
+    assert r.returncode == 2, r.stderr
+    # a good model: the generated directory
+    out = tmp_path / "gen"
+    r = ablc("-i", m, "-b", "hip", "-o", str(out))
+    assert r.returncode == 0, r.stderr
+    for f in ("main.cpp", "steps.hip", "abl_gen.h", "build.sh", "run.sh"):
+        assert (out / f).exists(), f
+    assert os.access(out / "build.sh", os.X_OK) and os.access(out / "run.sh", os.X_OK)
+
+
+def test_overload_names_and_folding(tmp_path):
+    """Overloads are printed name_k, k the number of earlier signatures of
+    the name (the standard library's random(float2, float2) is random_1
+    after the builtin random(float, float), random(float2) is random_4,
+    clamp(float2, float2) is clamp_4); folded constants print with 6
+    significant digits and '.0' when they show no dot; -P substitutes."""
+    out = tmp_path / "g"
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out))
+    assert r.returncode == 0, r.stderr
+    host = (out / "main.cpp").read_text()
+    assert "float2 random_1(float2 min, float2 max)" in host
+    assert "float2 random_4(float2 max)" in host
+    assert "float2 clamp_4(float2 pos, float2 max)" in host
+    assert "random_4(float2_fill((abl_float) (W)))" in host
+    assert "abl_float PI = 3.14159;" in host
+    assert "abl_float W = 141.421;" in host  # sqrt(1000 / 0.05) = 141.42135...
+    dev = (out / "steps.hip").read_text()
+    assert "static constexpr F W = (F) 141.421;" in dev
+    assert "clamp_4(p, v2_fill<F>((F) (W)))" in dev
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out),
+             "-P", "num_agents=1024", "-P", "rho=0.1024")
+    assert r.returncode == 0
+    host = (out / "main.cpp").read_text()
+    assert "int num_agents = 1024;" in host and "abl_float W = 100.0;" in host
+    # an int -P value for a float param is substituted as an int literal
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out), "-P", "r=3")
+    assert "abl_float r = 3;" in (out / "main.cpp").read_text()
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out), "-P", "r=x")
+    assert r.returncode == 1 and 'Value "x" provided for parameter "r" could not parsed' in r.stderr
+
+
+def test_float_literal_printing(tmp_path):
+    src = CIRCLE_MIN.replace("float W = sqrt(num_agents / rho);",
+                             "float W = sqrt(num_agents / rho);\nfloat tiny = 0.00001;\n"
+                             "float big = 1e3 * 1000.0;\nint folded = 7 / 2 + 1;")
+    # 1e3 lexes as the integer 1 followed by the identifier e3: not a float
+    r = ablc("--lint-only", "-i", model(tmp_path, src))
+    assert r.returncode == 1 and "Parse error: syntax error, unexpected IDENTIFIER" in r.stderr
+    src = src.replace("1e3 * 1000.0", "1.e3 * 1000.0")
+    out = tmp_path / "f"
+    r = ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out))
+    assert r.returncode == 0, r.stderr
+    host = (out / "main.cpp").read_text()
+    assert "abl_float tiny = 1.0e-05;" in host   # 1e-05, kept a C float literal
+    assert "abl_float big = 1.0e+06;" in host
+    assert "int folded = 4;" in host              # C integer division
+
+
+def test_step_program_shape(tmp_path):
+    """The device program of a step: the near loop over the grid with the C
+    near test, the written columns, the near radius as the program computes
+    it, registration under -C hip.program."""
+    out = tmp_path / "s"
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out),
+             "-C", "hip.program=circle_min", "-C", "hip.order=index")
+    assert r.returncode == 0, r.stderr
+    dev = (out / "steps.hip").read_text()
+    assert 'registers program "circle_min"' in dev and '"circle_min",' in dev
+    assert "c_.near(" in dev and "c_.outside(nx.pos, " in dev
+    assert "static __host__ __device__ double radius_move() { return (double) ((2.0 * r)); }" in dev
+    assert '{"move", 0, 0, nullptr, 0.0, 3u, launch_move, false, 0, false, radius_move}' in dev
+    host = (out / "main.cpp").read_text()
+    assert 'abl_gen_config(sim_, "hip.order", "index");' in host
+    assert 'abl_save_json(&agents, agents_info, "out.json", (int) sizeof(abl_float));' in host
+
+
+def test_copy_of_input_member_is_not_written(tmp_path):
+    """out.m = in.m alone leaves m's column as it is (copy-through), so a
+    step that only copies the position does not move the grid."""
+    src = """
+agent Cell { position float2 pos; bool alive; }
+environment { max: float2(10) }
+step s(Cell in -> out) {
+  int n = 0;
+  for (Cell c : near(in, 1.5)) { if (c.alive) n += 1; }
+  out.pos = in.pos;
+  out.alive = n == 3;
+}
+void main() {
+  for (int i : 0..10) { add(Cell { pos: float2(i, 0), alive: true }); }
+  simulate(3) { s }
+}
+"""
+    out = tmp_path / "c"
+    assert ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out)).returncode == 0
+    dev = (out / "steps.hip").read_text()
+    assert '{"s", 0, 0, nullptr, 0.0, 4u, launch_s,' in dev  # only the alive column
+
+
+def test_generated_main_links_step_library():
+    """The generated host program names no symbol of libsteps.so (the step
+    library registers its program with the runtime when it is loaded), so
+    the link keeps it against the linker's --as-needed default."""
+    gen = os.path.join(REPO, "openabl_amd", "gen", "golden_circle_f64")
+    if not os.path.exists(os.path.join(gen, "main")):
+        pytest.skip("generated programs not built")
+    assert "-Wl,--no-as-needed -lsteps" in open(os.path.join(gen, "build.sh")).read()
+    r = subprocess.run(["readelf", "-d", os.path.join(gen, "main")], capture_output=True, text=True)
+    assert "[libsteps.so]" in r.stdout
