@@ -14,13 +14,24 @@ skin and return at once otherwise.
 The JSON line carries
   value            agent-steps/s, FP64, whole timestep (grid + neighbour step);
   fp32             the same at use_float precision;
+  fp32_compensated the same at FP32-compensated precision, with its error
+                   (and FP32's) against the FP64 path over the first steps;
   roofline         the neighbour kernel: algorithmic bytes per launch
                    (N * 64 B + 4 * (cells + 1) for FP64) / its average duration,
-                   timed with HIP events on the simulation's own stream;
+                   timed with HIP events on the simulation's own stream; the
+                   HBM bytes the counters saw (traffic) come from the committed
+                   rocprofv3 passes of the same kernel sources, or are null;
+  valu_roofline    VALU issue rate of the same kernel from the committed SQ
+                   counters and this run's kernel duration (or null, as above);
   cpu_baseline     the reference C path (oracle/_ref: the restated generated
                    program linked against the reference's libabl.c, gcc -O2
                    -fopenmp, brute-force near loop) on a bounded sample: the
-                   first M agents' update_boid against the full 1M population.
+                   first M agents' update_boid against the full 1M population;
+  serving          p50/p99 run latency and aggregate agent-steps/s of a
+                   Poisson stream of mixed runs (1k-1M agents, 100-1000 steps);
+  ensemble         256 uneven boids2d members batched, against members alone;
+  precision_circle3d  1M clustered circle3d: FP64, FP32-compensated and FP32
+                   agent-steps/s and the per-step error against FP64.
 """
 from __future__ import annotations
 
@@ -36,6 +47,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GWAVE = 256 * 4 * 2.4 / 4  # G wave64 VALU instructions/s
 N_AGENTS = 1 << 20
 DENSITY = 0.065536
 PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.0,
@@ -52,6 +64,11 @@ PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.
 # search for any program)
 CONFIG = {"hip.cell_scale": 2, "hip.tile": 1, "hip.block": 256, "hip.lds_slack": 1.25,
           "hip.nlist": 1, "hip.skin": 0.05}
+
+
+def progress(msg: str) -> None:
+    """One line per finished leg on stderr (the JSON line is on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def neighbour_bytes(n: int, precision: str, cells: int) -> int:
@@ -79,18 +96,49 @@ def kernel_name(precision: str) -> str:
     return f"tiled_step<{pol}, {tx}, 8, 1, {rm}>"
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` under CONFIG from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/*_traffic.json, scaled by
-    a copy kernel of known size), or None when no pass matches."""
-    path = os.path.join(REPO, "profiles", "boids2d_1m_f64_traffic.json")
+# the sources the neighbour kernel is compiled from: a committed profile
+# figure is used only while these hash as they did when it was measured
+KERNEL_SOURCES = ("openabl_amd/csrc/abl_tile.hpp", "openabl_amd/csrc/abl_device.hpp",
+                  "openabl_amd/csrc/abl_pipeline.hpp", "openabl_amd/csrc/abl_runtime.hpp",
+                  "openabl_amd/csrc/prog_boids2d.hip")
+
+
+def kernel_source_sha() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(REPO, rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_figures(kernel: str, precision: str = "f64"):
+    """The committed rocprofv3 counter figures of `kernel` under CONFIG
+    (profiles/boids2d_1m_<prec>_counters.json: FETCH_SIZE/WRITE_SIZE passes
+    scaled by a copy kernel of known size, and the SQ instruction counters),
+    or (None, reason) when none match this kernel, launch configuration and
+    kernel sources."""
+    path = os.path.join(REPO, "profiles", f"boids2d_1m_{precision}_counters.json")
     if not os.path.exists(path):
-        return None
+        return None, "no committed counter profile"
     with open(path) as f:
         t = json.load(f)
     if t.get("kernel") != kernel or t.get("launch_config") != CONFIG:
-        return None
-    return int(t["traffic_bytes"])
+        return None, "committed profile is of another kernel or launch configuration"
+    if t.get("source_sha16") != kernel_source_sha():
+        return None, "committed profile predates the current kernel sources"
+    return t, f"committed profile {os.path.relpath(path, REPO)} (source sha {t['source_sha16']})"
+
+
+def cpu_model() -> str | None:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def grid_cells(box: float, granularity: float, cell_scale: int) -> int:
@@ -134,6 +182,38 @@ def run_gpu(precision: str, steps: int, warmup: int, timing_steps: int):
     }
 
 
+def boids_precision_error(steps: int):
+    """FP32-compensated and FP32 (use_float) boids2d on the metric workload,
+    from the FP64 path's initial state, against the FP64 path (which the
+    parity tests hold to the oracle within 1e-12 per step): the largest
+    position and velocity differences after each step."""
+    import numpy as np
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    dt = record_dtype(MEMBERS["boids2d"], "f64")
+    dt32 = record_dtype(MEMBERS["boids2d"], "f32")
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    sims = {"f64": Simulation("boids2d", "f64", params=PARAMS, config=CONFIG)}
+    sims["f64"].init_agents()
+    s0 = sims["f64"].download(dt)
+    for prec in ("f32c", "f32"):
+        sims[prec] = Simulation("boids2d", prec, params=PARAMS, config=CONFIG)
+        sims[prec].upload(s0 if prec == "f32c" else s0.astype(dt32))
+    err = {p: {"pos_max_abs": [], "vel_max_abs": []} for p in ("f32c", "f32")}
+    for _ in range(steps):
+        for sim in sims.values():
+            sim.simulate(1)
+        want = sims["f64"].download(dt)
+        for prec in ("f32c", "f32"):
+            got = sims[prec].download(dt if prec == "f32c" else dt32)
+            for key, col in (("pos_max_abs", "pos"), ("vel_max_abs", "velocity")):
+                d = np.abs(got[col].astype(np.float64) - want[col]).max()
+                err[prec][key].append(float(f"{d:.3e}"))
+    for sim in sims.values():
+        sim.close()
+    return {"steps": steps, "reference": "the FP64 path from the same initial state",
+            "box": 4000.0, "max_speed": PARAMS["max_speed"], **err}
+
+
 def cpu_baseline(sample_agents: int):
     exe = os.path.join(REPO, "oracle", "_ref", "boids2d_f64")
     kind = "reference"
@@ -149,6 +229,7 @@ def cpu_baseline(sample_agents: int):
     kv = dict(tok.split("=") for tok in out.split())
     secs = float(kv["sample_seconds"])
     return {"value": sample_agents / secs, "unit": "agent-steps/s", "cores": cores, "kind": kind,
+            "cpu_model": cpu_model(),
             "sample": (f"update_boid for the first {sample_agents} of {N_AGENTS} agents, each "
                        f"scanning all {N_AGENTS} (the C backend's brute-force near loop, "
                        f"gcc -O2 -std=c99 -fopenmp), {secs:.2f} s"),
@@ -210,11 +291,11 @@ def run_ensemble(args):
                      "device_ms_per_step": round(dev_ms / args.ens_steps, 5),
                      "alone_sequential_agent_steps_per_s": round(alone_rate, 1),
                      "speedup_vs_alone": round(total * args.ens_steps / wall / alone_rate, 2)}
-    print(json.dumps({"metric": "batched ensemble agent-steps/s (256 boids2d members)",
-                      "members": len(ms), "agents": total, "steps": args.ens_steps,
-                      "config": "N log-uniform [1k, 64k], box sqrt(N/0.065), weights U[0.5,2], "
-                                "cell_scale 2, 16x8 tiles, graph-replayed timesteps",
-                      **res}))
+    return {"metric": "batched ensemble agent-steps/s (256 boids2d members)",
+            "members": len(ms), "agents": total, "steps": args.ens_steps,
+            "config": "N log-uniform [1k, 64k], box sqrt(N/0.065), weights U[0.5,2], "
+                      "cell_scale 2, 16x8 tiles, graph-replayed timesteps",
+            **res}
 
 
 CIRCLE3D_PARAMS = dict(num_agents=1 << 20, rho=(1 << 20) / 1e9, r=4.0, k_rep=0.05, k_att=0.01,
@@ -277,7 +358,7 @@ def run_precision(args):
             rel_err[prec].append(float((d / np.maximum(np.abs(a), 1.0)).max()))
     for s in sims.values():
         s.close()
-    print(json.dumps({
+    return {
         "metric": "agent-steps/s, 1M circle3d clustered, FP32-compensated vs FP64",
         "value": res["f32c"]["agent_steps_per_s"], "unit": "agent-steps/s",
         "f64": res["f64"], "f32c": res["f32c"], "f32": res["f32"],
@@ -292,7 +373,7 @@ def run_precision(args):
         "config": {"workload": "circle3d_clustered 1,048,576 agents, 1000^3 box, 64 clusters "
                                "sigma 20 + 20% uniform, radius 8", "steps": args.prec_steps,
                    "launch_config": CIRCLE3D_CONFIG},
-    }))
+    }
 
 
 def run_serving(args):
@@ -334,17 +415,17 @@ def run_serving(args):
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    print(json.dumps({"metric": "serving: p50/p99 run latency and agent-steps/s under load",
-                      "runs": len(reqs), "p50_s": round(st["p50_s"], 4),
-                      "p99_s": round(st["p99_s"], 4),
-                      "agent_steps_per_s": round(st["agent_steps"] / wall, 1),
-                      "wall_s": round(wall, 3), "peak_admitted_bytes": st["peak_bytes"],
-                      "config": {"mix": "circle f64, boids2d f32, game_of_life, predator_prey",
-                                 "agents": f"log-uniform 1k-{args.serve_max_agents}",
-                                 "steps": "uniform 100-1000",
-                                 "mean_interarrival_s": args.serve_interval,
-                                 "budget_gb": args.serve_budget_gb,
-                                 "max_active": args.serve_active}}))
+    return {"metric": "serving: p50/p99 run latency and agent-steps/s under load",
+            "runs": len(reqs), "p50_s": round(st["p50_s"], 4),
+            "p99_s": round(st["p99_s"], 4),
+            "agent_steps_per_s": round(st["agent_steps"] / wall, 1),
+            "wall_s": round(wall, 3), "peak_admitted_bytes": st["peak_bytes"],
+            "config": {"mix": "circle f64, boids2d f32, game_of_life, predator_prey",
+                       "agents": f"log-uniform 1k-{args.serve_max_agents}",
+                       "steps": "uniform 100-1000",
+                       "mean_interarrival_s": args.serve_interval,
+                       "budget_gb": args.serve_budget_gb,
+                       "max_active": args.serve_active}}
 
 
 def main():
@@ -356,6 +437,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the runs, copy 64 MiB device to device 5 times (the HIP "
                          "runtime's copy kernel: reference for scaling HBM byte counters)")
@@ -374,22 +459,43 @@ def main():
               "(the 1M-agent workload fits one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.mode == "ensemble":
-        run_ensemble(args)
+        print(json.dumps(run_ensemble(args)))
         return
     if args.mode == "serving":
-        run_serving(args)
+        print(json.dumps(run_serving(args)))
         return
     if args.mode == "precision":
-        run_precision(args)
+        print(json.dumps(run_precision(args)))
         return
 
     f64 = run_gpu("f64", args.steps, args.warmup, args.timing_steps)
+    progress(f"f64: {f64['agent_steps_per_s']:.4g} agent-steps/s")
     f32 = None if args.no_fp32 else run_gpu("f32", args.steps, args.warmup, args.timing_steps)
     f32c = None if args.no_fp32 else run_gpu("f32c", args.steps, args.warmup, args.timing_steps)
+    if f32c:
+        progress(f"f32: {f32['agent_steps_per_s']:.4g}, f32c: {f32c['agent_steps_per_s']:.4g}")
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample)
+    if cpu:
+        progress(f"cpu baseline: {cpu['value']:.4g} agent-steps/s")
 
     b = neighbour_bytes(N_AGENTS, "f64", f64["cells"])
     achieved = b / (f64["nbr_ms_avg"] * 1e-3) / 1e9
+    kernel = kernel_name("f64")
+    figures, source = profile_figures(kernel)
+    valu = None
+    if figures and "sq" in figures:
+        # VALU issue: wave64 instructions per launch over the kernel's
+        # duration, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
+        # instruction (FP64 and non-packed FP32 issue at that rate on MI355X)
+        sq = figures["sq"]
+        rate = sq["SQ_INSTS_VALU"] / (f64["nbr_ms_avg"] * 1e-3) / 1e9
+        peak = VALU_PEAK_GWAVE
+        valu = {"bound": "valu", "achieved": round(rate, 1), "peak": peak,
+                "unit": "G wave64-instructions/s", "frac": round(rate / peak, 4),
+                "lanes_active": round(sq["SQ_THREAD_CYCLES_VALU"] /
+                                      (sq["SQ_ACTIVE_INST_VALU"] * 64.0), 3),
+                "valu_per_wave": round(sq["SQ_INSTS_VALU"] / sq["SQ_WAVES"], 1),
+                "source": source}
     line = {
         "metric": "agent-steps/sec, 1M boids2d FP64 vs FP32",
         "launch_config": CONFIG,
@@ -410,9 +516,11 @@ def main():
                    "step": "grid rebuild + update_boid"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(kernel_name("f64")),
-                     "kernel": kernel_name("f64"),
+                     "traffic": figures and figures["traffic_bytes"],
+                     "traffic_source": source,
+                     "kernel": kernel,
                      "bytes_per_launch": b, "avg_kernel_ms": round(f64["nbr_ms_avg"], 5)},
+        "valu_roofline": valu or {"source": source},
         "grid_ms_per_step": round(f64["grid_ms_avg"], 5),
         "cpu_baseline": cpu and {k: v for k, v in cpu.items() if k != "seconds"},
     }
@@ -424,6 +532,18 @@ def main():
                          "avg_kernel_ms": round(r["nbr_ms_avg"], 5),
                          "grid_ms_per_step": round(r["grid_ms_avg"], 5),
                          "kernel_GBs": round(bp / (r["nbr_ms_avg"] * 1e-3) / 1e9, 1)}
+    if f32c and args.error_steps > 0:
+        line["fp32_compensated"]["error_vs_f64"] = boids_precision_error(args.error_steps)
+        progress("f32c error leg done")
+    if not args.no_ensemble:
+        line["ensemble"] = run_ensemble(args)
+        progress("ensemble leg done")
+    if not args.no_precision:
+        line["precision_circle3d"] = run_precision(args)
+        progress("circle3d precision leg done")
+    if not args.no_serving:
+        line["serving"] = run_serving(args)
+        progress("serving leg done")
     print(json.dumps(line))
     if args.copy_probe:
         import torch

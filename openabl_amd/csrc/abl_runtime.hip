@@ -222,6 +222,12 @@ struct abl_sim {
   bool fused_rebuild = true;  // "hip.rebuild": gated re-sort + list build in one launch
   uint32_t barrier_spins = kPhaseSpins;   // "hip.barrier_spins": spin limit of its phase waits
   unsigned rebuild_blocks = 0;  // "hip.rebuild_blocks": its launch size (0: resident capacity)
+  // "hip.defer_upload": abl_hip_upload keeps the records on the host (no
+  // device call), and turning it off uploads what was kept
+  bool defer_upload = false;
+  std::vector<std::vector<char>> pending;  // per agent type
+  std::vector<size_t> pending_n;
+  std::vector<char> has_pending;
   bool timing = false;
   hipEvent_t ev[6] = {};
   double last_ms[3] = {0, 0, 0};
@@ -969,6 +975,25 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     const long long nb = std::atoll(value);
     if (nb < 0 || nb > 0xFFF8) return fail(ABL_ERR_ARG, "hip.rebuild_blocks must be 0..65528");
     sim->rebuild_blocks = (unsigned) nb;
+  } else if (k == "hip.defer_upload") {
+    if (v == "1") {
+      sim->defer_upload = true;
+      sim->pending.assign(sim->types.size(), {});
+      sim->pending_n.assign(sim->types.size(), 0);
+      sim->has_pending.assign(sim->types.size(), 0);
+    } else if (v == "0") {
+      sim->defer_upload = false;
+      for (size_t t = 0; t < sim->has_pending.size(); t++) {
+        if (!sim->has_pending[t]) continue;
+        const int rc = abl_hip_upload(sim, (int) t, sim->pending[t].data(), sim->pending_n[t]);
+        if (rc) return rc;
+      }
+      sim->pending.clear();
+      sim->pending_n.clear();
+      sim->has_pending.clear();
+    } else {
+      return fail(ABL_ERR_ARG, "hip.defer_upload must be 0 or 1");
+    }
   } else if (k == "hip.seed") {
     sim->seed = std::strtoull(value, nullptr, 0);
   } else if (k == "hip.log_csv") {
@@ -1013,6 +1038,14 @@ extern "C" int abl_hip_upload(abl_sim *sim, int t, const void *aos, size_t count
   if (count && !aos) return fail(ABL_ERR_ARG, "null agent data");
   if (count > (size_t) INT32_MAX) return fail(ABL_ERR_ARG, "too many agents");
   AgentStore &s = sim->types[t];
+  if (sim->defer_upload) {
+    // kept on the host until hip.defer_upload is turned off (no device call)
+    const char *src = static_cast<const char *>(aos);
+    sim->pending[(size_t) t].assign(src, src + count * s.aos_size);
+    sim->pending_n[(size_t) t] = count;
+    sim->has_pending[(size_t) t] = 1;
+    return ABL_OK;
+  }
   int rc = reserve_store(sim, s, count);
   if (rc) return rc;
   s.n = count;

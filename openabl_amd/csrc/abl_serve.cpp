@@ -2,24 +2,28 @@
 //
 // Requests (a program, its parameters, a timestep count, the generator
 // state its initialisation starts from) arrive at any time through
-// abl_server_submit.  Three threads serve them:
-//   issuer    admits queued requests in arrival order while the estimated
-//             device footprint of the admitted runs stays within the HBM
-//             budget and their count within `max_active`, and advances every
-//             ready run `chunk_steps` timesteps at a time on that run's own
-//             stream (a launch queue, never a wait), so runs overlap on the
-//             device;
-//   starter   builds each admitted run's simulation and initial state on the
-//             host and uploads it (the generator is process-wide, so the
-//             initialisations are serialised on this one thread);
-//   finisher  downloads each finished run (reference array order), keeps its
-//             records and log, and releases its device memory.
-// Host initialisation and download therefore never hold up issuing work to
-// the runs already on the device.  A run's state, error text, timestamps
-// and results are published together under the server's mutex, which is
-// also how every reader takes them.  Each run is computed exactly as it
-// would be alone: runs share no device state, and agent order inside a run
-// depends only on that run's positions and ids.
+// abl_server_submit.  Two threads serve them:
+//   issuer    the only thread that drives the device.  It admits queued
+//             requests in arrival order while the estimated device footprint
+//             of the admitted runs stays within the HBM budget and their
+//             count within `max_active`, creates each admitted run's
+//             simulation, uploads the initial state the initialiser built,
+//             advances every ready run `chunk_steps` timesteps at a time on
+//             that run's own stream (a launch queue, never a wait), so runs
+//             overlap on the device, and downloads and releases each finished
+//             run;
+//   starter   builds each created run's initial state on the host (the
+//             model's initialisation with the run's generator state; the
+//             simulation holds the records back under hip.defer_upload, so
+//             this thread makes no device call; the generator is
+//             process-wide, so initialisations are serialised here).
+// The costly host part of starting a run therefore never holds up issuing
+// work to the runs already on the device, and every device call of the
+// server comes from one thread.  A run's state, error text, timestamps and
+// results are published together under the server's mutex, which is also
+// how every reader takes them.  Each run is computed exactly as it would be
+// alone: runs share no device state, and agent order inside a run depends
+// only on that run's positions and ids.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -39,9 +43,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// Public states (abl_hip.h) plus the two hand-over states, reported as ACTIVE.
-enum { RUN_QUEUED = 0, RUN_ACTIVE = 1, RUN_DONE = 2, RUN_FAILED = 3, RUN_STARTING = 4,
-       RUN_FINISHING = 5 };
+// Public states (abl_hip.h) plus the hand-over state, reported as ACTIVE.
+enum { RUN_QUEUED = 0, RUN_ACTIVE = 1, RUN_DONE = 2, RUN_FAILED = 3, RUN_STARTING = 4 };
 
 struct Run {
   uint64_t id = 0;
@@ -56,6 +59,8 @@ struct Run {
   int state = RUN_QUEUED;  // guarded by abl_server::mu
   std::string error;       // guarded by abl_server::mu
   abl_sim *sim = nullptr;  // owned by the thread that holds the run
+  bool init_ok = false;    // starter -> issuer
+  std::string init_error;  // starter -> issuer
   int issued = 0;          // timesteps handed to the device so far (issuer)
   size_t agents = 0;
   Clock::time_point t_submit, t_admit, t_done;
@@ -69,17 +74,16 @@ struct Run {
 struct abl_server {
   std::mutex mu;
   std::condition_variable cv;       // waiters: a run ended
-  std::condition_variable work_cv;  // issuer, starter, finisher: work arrived
+  std::condition_variable work_cv;  // issuer, starter: work arrived
   std::deque<Run *> queue;          // submitted, not admitted
-  std::deque<Run *> to_start;       // admitted, waiting for the starter
-  std::vector<Run *> ready;         // started, held by the issuer
-  std::deque<Run *> to_finish;      // all timesteps done, waiting for the finisher
+  std::deque<Run *> to_init;        // created, waiting for the starter
+  std::vector<Run *> inited;        // initial state built, back to the issuer
   std::map<uint64_t, Run *> runs;
-  std::thread issuer, starter, finisher;
+  std::thread issuer, starter;
   bool stop = false;
   size_t budget = 0, used = 0, peak = 0;
   int max_active = 8;
-  int admitted = 0;  // runs holding budget (starting, ready or finishing)
+  int admitted = 0;  // runs holding budget (starting or ready)
   int chunk = 20;
   uint64_t next_id = 1;
   double agent_steps = 0.0;
@@ -113,8 +117,7 @@ void end_run_locked(abl_server *s, Run *r, int state, const std::string &error) 
   s->work_cv.notify_all();
 }
 
-// A failed run's message and device memory (outside the lock, by the thread
-// that holds the run).
+// A failed run's message; its device memory is released (issuer).
 std::string drop_sim(Run *r, const std::string &what) {
   std::string msg = what + ": " + abl_hip_last_error();
   if (r->sim) {
@@ -124,8 +127,9 @@ std::string drop_sim(Run *r, const std::string &what) {
   return msg;
 }
 
-// Build the run's simulation and initial state (starter thread).
-bool start_run(Run *r, std::string *err) {
+// The run's simulation with its parameters and configuration, holding
+// uploads back on the host (issuer).
+bool create_run(Run *r, std::string *err) {
   if (abl_hip_create(r->program.c_str(), r->precision, &r->sim)) {
     *err = drop_sim(r, "create");
     return false;
@@ -142,9 +146,29 @@ bool start_run(Run *r, std::string *err) {
       return false;
     }
   }
+  if (abl_hip_set_config(r->sim, "hip.defer_upload", "1")) {
+    *err = drop_sim(r, "config hip.defer_upload");
+    return false;
+  }
+  return true;
+}
+
+// The initial state, on the host only (starter).
+void init_run(Run *r) {
   abl_rng_seed(r->rng[0], r->rng[1]);
-  if (abl_hip_init_agents(r->sim)) {
-    *err = drop_sim(r, "init");
+  r->init_ok = abl_hip_init_agents(r->sim) == ABL_OK;
+  if (!r->init_ok) r->init_error = std::string("init: ") + abl_hip_last_error();
+}
+
+// Upload the initial state the starter built (issuer).
+bool upload_run(Run *r, std::string *err) {
+  if (!r->init_ok) {
+    *err = r->init_error;
+    drop_sim(r, "init");
+    return false;
+  }
+  if (abl_hip_set_config(r->sim, "hip.defer_upload", "0")) {
+    *err = drop_sim(r, "upload");
     return false;
   }
   r->agents = 0;
@@ -156,7 +180,7 @@ bool start_run(Run *r, std::string *err) {
   return true;
 }
 
-// Download the results and release the device memory (finisher thread).
+// Download the results and release the device memory (issuer).
 bool finish_run(Run *r, std::string *err) {
   if (abl_hip_synchronize(r->sim)) {
     *err = drop_sim(r, "synchronize");
@@ -191,8 +215,9 @@ bool finish_run(Run *r, std::string *err) {
 }
 
 // Admission (caller holds s->mu): in arrival order while memory and slots
-// allow; a run that can never fit fails at once.
-void admit_locked(abl_server *s) {
+// allow; a run that can never fit fails at once.  Admitted runs are handed
+// back to the issuer for creation.
+void admit_locked(abl_server *s, std::vector<Run *> *admitted) {
   while (!s->queue.empty() && s->admitted < s->max_active) {
     Run *r = s->queue.front();
     if (s->used + r->footprint > s->budget) {
@@ -215,64 +240,61 @@ void admit_locked(abl_server *s) {
       s->any_admitted = true;
       s->t_first_admit = r->t_admit;
     }
-    s->to_start.push_back(r);
-    s->work_cv.notify_all();
+    admitted->push_back(r);
   }
 }
 
 void starter_loop(abl_server *s) {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    s->work_cv.wait(lk, [&] { return s->stop || !s->to_start.empty(); });
+    s->work_cv.wait(lk, [&] { return s->stop || !s->to_init.empty(); });
     if (s->stop) return;
-    Run *r = s->to_start.front();
-    s->to_start.pop_front();
+    Run *r = s->to_init.front();
+    s->to_init.pop_front();
     lk.unlock();
-    std::string err;
-    const bool ok = start_run(r, &err);
+    init_run(r);
     lk.lock();
-    if (ok) {
-      r->state = RUN_ACTIVE;
-      s->ready.push_back(r);
-      s->work_cv.notify_all();
-    } else {
-      end_run_locked(s, r, RUN_FAILED, err);
-    }
+    s->inited.push_back(r);
+    s->work_cv.notify_all();
   }
 }
 
-void finisher_loop(abl_server *s) {
-  std::unique_lock<std::mutex> lk(s->mu);
-  for (;;) {
-    s->work_cv.wait(lk, [&] { return s->stop || !s->to_finish.empty(); });
-    if (s->stop) return;
-    Run *r = s->to_finish.front();
-    s->to_finish.pop_front();
-    lk.unlock();
-    std::string err;
-    const bool ok = finish_run(r, &err);
-    lk.lock();
-    end_run_locked(s, r, ok ? RUN_DONE : RUN_FAILED, err);
-  }
-}
-
-// The issuer: admission, then a pass over the started runs: each run whose
-// stream is idle gets its next chunk of timesteps, or goes to the finisher.
+// The issuer: admission and creation, uploads of initialised runs, then a
+// pass over the started runs: each run whose stream is idle gets its next
+// chunk of timesteps, or is downloaded and released.
 void issuer_loop(abl_server *s) {
+  std::vector<Run *> ready;  // started runs, held by this thread only
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop) {
-    admit_locked(s);
-    std::vector<Run *> mine;
-    mine.swap(s->ready);
+    std::vector<Run *> fresh, uploaded;
+    admit_locked(s, &fresh);
+    uploaded.swap(s->inited);
     lk.unlock();
 
-    bool progressed = false;
-    std::vector<Run *> keep, finished;
-    std::vector<std::pair<Run *, std::string>> failed;
-    for (Run *r : mine) {
+    bool progressed = !fresh.empty() || !uploaded.empty();
+    std::vector<Run *> created;
+    std::vector<std::pair<Run *, std::string>> ended;  // (run, error: empty = done)
+    for (Run *r : fresh) {
+      std::string err;
+      if (create_run(r, &err)) created.push_back(r);
+      else ended.push_back({r, err.empty() ? "create failed" : err});
+    }
+    for (Run *r : uploaded) {
+      std::string err;
+      if (upload_run(r, &err)) {
+        lk.lock();
+        r->state = RUN_ACTIVE;
+        lk.unlock();
+        ready.push_back(r);
+      } else {
+        ended.push_back({r, err.empty() ? "init failed" : err});
+      }
+    }
+    std::vector<Run *> keep;
+    for (Run *r : ready) {
       int done = 0;
       if (abl_hip_query(r->sim, &done)) {
-        failed.push_back({r, drop_sim(r, "query")});
+        ended.push_back({r, drop_sim(r, "query")});
         continue;
       }
       if (!done) {
@@ -280,39 +302,39 @@ void issuer_loop(abl_server *s) {
         continue;
       }
       if (r->issued >= r->timesteps) {
-        finished.push_back(r);
+        std::string err;
+        ended.push_back({r, finish_run(r, &err) ? std::string() : (err.empty() ? "finish failed" : err)});
+        progressed = true;
         continue;
       }
       const int k = std::min(s->chunk, r->timesteps - r->issued);
       if (abl_hip_simulate(r->sim, k)) {
-        failed.push_back({r, drop_sim(r, "simulate")});
+        ended.push_back({r, drop_sim(r, "simulate")});
         continue;
       }
       r->issued += k;
       progressed = true;
       keep.push_back(r);
     }
+    ready.swap(keep);
 
     lk.lock();
-    s->ready.insert(s->ready.end(), keep.begin(), keep.end());
-    for (Run *r : finished) {
-      r->state = RUN_FINISHING;
-      s->to_finish.push_back(r);
-      progressed = true;
-    }
-    if (!finished.empty()) s->work_cv.notify_all();
-    for (auto &f : failed) end_run_locked(s, f.first, RUN_FAILED, f.second);
+    for (Run *r : created) s->to_init.push_back(r);
+    if (!created.empty()) s->work_cv.notify_all();
+    for (auto &e : ended) end_run_locked(s, e.first, e.second.empty() ? RUN_DONE : RUN_FAILED, e.second);
     if (!progressed) s->work_cv.wait_for(lk, std::chrono::microseconds(200));
   }
+  lk.unlock();
+  // stopping: release the runs this thread still holds (device calls stay
+  // on this thread)
+  for (Run *r : ready) drop_sim(r, "stopped");
 }
 
 double seconds(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double>(b - a).count();
 }
 
-int public_state(int st) {
-  return st == RUN_STARTING || st == RUN_FINISHING ? RUN_ACTIVE : st;
-}
+int public_state(int st) { return st == RUN_STARTING ? RUN_ACTIVE : st; }
 
 }  // namespace
 
@@ -327,7 +349,6 @@ extern "C" int abl_server_create(size_t hbm_budget, int max_active, int chunk_st
   s->chunk = chunk_steps;
   s->issuer = std::thread(issuer_loop, s);
   s->starter = std::thread(starter_loop, s);
-  s->finisher = std::thread(finisher_loop, s);
   *out = s;
   return ABL_OK;
 }
@@ -476,7 +497,8 @@ extern "C" int abl_server_destroy(abl_server *s) {
   }
   s->issuer.join();
   s->starter.join();
-  s->finisher.join();
+  // runs the issuer had not taken back from the starter (their simulations
+  // hold no device work: both threads have ended)
   for (auto &kv : s->runs) {
     if (kv.second->sim) abl_hip_destroy(kv.second->sim);
     delete kv.second;
