@@ -223,6 +223,128 @@ __device__ __forceinline__ void for_each_candidate(const GridView &g, int cx, in
   }
 }
 
+// Conservative x-interval of cells, relative to the agent's own cell, that
+// can hold agents within `rc` cell units of an agent at fractional in-cell
+// position (ux, uy[, uz]) when looking at the row of cells dy (, dz) away.
+// Returns false when the whole row is out of reach.  Positions outside
+// [0, 1) (agents clamped into edge cells) disable narrowing.
+__device__ __forceinline__ bool row_interval(float ux, float uy, float uz, int dy, int dz,
+                                             float rc, int reach, int *xa, int *xb) {
+  const bool inside = ux >= 0.f && ux < 1.f && uy >= 0.f && uy < 1.f && uz >= 0.f && uz < 1.f;
+  if (!inside) {
+    *xa = -reach;
+    *xb = reach;
+    return true;
+  }
+  const float gy = dy > 0 ? (float) dy - uy : (dy < 0 ? uy + (float) (-dy) - 1.f : 0.f);
+  const float gz = dz > 0 ? (float) dz - uz : (dz < 0 ? uz + (float) (-dz) - 1.f : 0.f);
+  const float gap2 = gy * gy + gz * gz;
+  // radius squared, enlarged well past the float error of gap2 (values <= ~16,
+  // error ~1e-6) so every agent the exact test can accept stays inside
+  const float margin = 1e-4f;
+  const float rr2 = rc * rc * (1.f + 1e-5f) + margin;
+  if (gap2 > rr2) return false;
+  const float w = __builtin_sqrtf(rr2 - gap2) + margin;
+  int a = (int) floorf(ux - w);
+  int b = (int) floorf(ux + w);
+  *xa = a < -reach ? -reach : a;
+  *xb = b > reach ? reach : b;
+  return *xa <= *xb;
+}
+
+// The stencil rows of cells around the agent at (px, py, pz) narrowed to
+// the cells that can hold agents within rc cell units (row_interval): row(a,
+// b) for each non-empty range [a, b) of the cell-sorted order, rows in the
+// order of for_each_candidate.  Only cells that cannot hold a neighbour are
+// skipped, so a visit that applies the exact near test to every candidate
+// accepts the same agents in the same order.
+template <typename Row>
+__device__ __forceinline__ void for_each_candidate_row(const GridView &g, double px, double py,
+                                                       double pz, float rc, Row &&row) {
+  const int cx = cell_coord(px, g.lo[0], g.inv_cell, g.n[0]);
+  const int cy = cell_coord(py, g.lo[1], g.inv_cell, g.n[1]);
+  const int cz = g.dim == 3 ? cell_coord(pz, g.lo[2], g.inv_cell, g.n[2]) : 0;
+  const float ux = (float) ((px - g.lo[0]) * g.inv_cell - (double) cx);
+  const float uy = (float) ((py - g.lo[1]) * g.inv_cell - (double) cy);
+  const float uz = g.dim == 3 ? (float) ((pz - g.lo[2]) * g.inv_cell - (double) cz) : 0.f;
+  const int R = g.reach;
+  const int zr = g.dim == 3 ? R : 0;
+  for (int dz = -zr; dz <= zr; dz++) {
+    const int z = cz + dz;
+    if (z < 0 || (g.dim == 3 && z >= g.n[2])) continue;
+    for (int dy = -R; dy <= R; dy++) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.n[1]) continue;
+      int xa, xb;
+      if (!row_interval(ux, uy, uz, dy, dz, rc, R, &xa, &xb)) continue;
+      const int x0 = max(cx + xa, 0);
+      const int x1 = min(cx + xb, g.n[0] - 1);
+      if (x0 > x1) continue;
+      const uint32_t base = ((uint32_t) z * (uint32_t) g.n[1] + (uint32_t) y) * (uint32_t) g.n[0];
+      const uint32_t a = g.start[base + x0];
+      const uint32_t b = g.start[base + x1 + 1];
+      if (a < b) row(a, b);
+    }
+  }
+}
+
+// A column read through a buffer resource: the column's base address is
+// scalar and each lane supplies a 32-bit byte offset, so one offset serves
+// every column of a record (the record's columns are at the same offset) and
+// a read past `bytes` returns zero.  The resource's last word is the one the
+// gfx9 family takes for untyped (raw) buffer access.
+struct ColBuf {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ ColBuf col_buf(const void *p, uint32_t bytes) {
+  return {__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int) bytes, 0x00020000)};
+}
+template <typename F>
+__device__ __forceinline__ F col_load(const ColBuf &b, uint32_t off) {
+  if constexpr (sizeof(F) == 8) {
+    return __builtin_bit_cast(F, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int) off, 0, 0));
+  } else {
+    return __builtin_bit_cast(F, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int) off, 0, 0));
+  }
+}
+
+// ---- quotients with a shared divisor --------------------------------------------
+// a / b correctly rounded from y = RN(1 / b) with three more operations:
+// q = RN(a y) is within an ulp of a / b, the residual a - b q is exact in one
+// fma, and RN(q + (a - b q) y) is a / b correctly rounded (Markstein's
+// theorem: P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Develop. 34(1), 1990; the final
+// step of Newton-Raphson division in J.-M. Muller et al., Handbook of
+// Floating-Point Arithmetic).  The theorem needs the quotient and the
+// residual clear of underflow and overflow: div_ok() is the magnitude range
+// that guarantees it (it also excludes zeros, whose sign the residual step
+// can lose, and non-finite values); outside it the IEEE division is used.
+// Quotients that share a divisor then cost one division in all.
+template <typename F>
+struct DivRange;
+template <>
+struct DivRange<double> {
+  static constexpr double lo = 0x1p-400, hi = 0x1p400;
+};
+template <>
+struct DivRange<float> {
+  static constexpr float lo = 0x1p-50f, hi = 0x1p50f;
+};
+template <typename F>
+__device__ __forceinline__ bool div_ok(F x) {
+  const F ax = x < (F) 0 ? -x : x;
+  return ax >= DivRange<F>::lo && ax <= DivRange<F>::hi;
+}
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// a / b given y = RN(1 / b), valid when div_ok(a) and div_ok(b)
+template <typename F>
+__device__ __forceinline__ F div_by_recip(F a, F b, F y) {
+  const F q = a * y;
+  const F r = fma_rn(-b, q, a);
+  return fma_rn(r, y, q);
+}
+
 // Same candidates in ascending id order (the reference's visiting order):
 // ids ascend inside every cell, so a merge over the stencil's cells yields
 // the global id order.

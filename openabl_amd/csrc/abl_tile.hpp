@@ -227,35 +227,6 @@ __host__ __device__ inline unsigned tile_count(const GridView &g) {
   return n;
 }
 
-// Conservative x-interval of cells, relative to the agent's own cell, that
-// can hold agents within `rc` cell units of an agent at fractional in-cell
-// position (ux, uy[, uz]) when looking at the row of cells dy (, dz) away.
-// Returns false when the whole row is out of reach.  Positions outside
-// [0, 1) (agents clamped into edge cells) disable narrowing.
-__device__ __forceinline__ bool row_interval(float ux, float uy, float uz, int dy, int dz,
-                                             float rc, int reach, int *xa, int *xb) {
-  const bool inside = ux >= 0.f && ux < 1.f && uy >= 0.f && uy < 1.f && uz >= 0.f && uz < 1.f;
-  if (!inside) {
-    *xa = -reach;
-    *xb = reach;
-    return true;
-  }
-  const float gy = dy > 0 ? (float) dy - uy : (dy < 0 ? uy + (float) (-dy) - 1.f : 0.f);
-  const float gz = dz > 0 ? (float) dz - uz : (dz < 0 ? uz + (float) (-dz) - 1.f : 0.f);
-  const float gap2 = gy * gy + gz * gz;
-  // radius squared, enlarged well past the float error of gap2 (values <= ~16,
-  // error ~1e-6) so every agent the exact test can accept stays inside
-  const float margin = 1e-4f;
-  const float rr2 = rc * rc * (1.f + 1e-5f) + margin;
-  if (gap2 > rr2) return false;
-  const float w = __builtin_sqrtf(rr2 - gap2) + margin;
-  int a = (int) floorf(ux - w);
-  int b = (int) floorf(ux + w);
-  *xa = a < -reach ? -reach : a;
-  *xb = b > reach ? reach : b;
-  return *xa <= *xb;
-}
-
 }  // namespace abl
 
 namespace abl {

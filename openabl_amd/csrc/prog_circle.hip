@@ -61,8 +61,20 @@ __device__ __forceinline__ void circle_visit(const CircleArgs<F, DIM> &a, const 
   F force[3];
   if ((double) sep_dist > 0.0) {
     const F s = (F) ((double) a.k_att * ((2.0 * (double) a.r) - (double) pos_dist));
+    // the DIM quotients by pos_dist through one reciprocal (IEEE results)
+    F num[3];
+    const F y = (F) 1 / pos_dist;
+    bool ok = div_ok(pos_dist);
 #pragma unroll
-    for (int d = 0; d < DIM; d++) force[d] = ((q[d] - x[d]) * s) / pos_dist;
+    for (int d = 0; d < DIM; d++) {
+      num[d] = (q[d] - x[d]) * s;
+      force[d] = div_by_recip(num[d], pos_dist, y);
+      ok = ok && div_ok(num[d]);
+    }
+    if (__builtin_expect(!ok, 0)) {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) force[d] = num[d] / pos_dist;
+    }
   } else {
 #pragma unroll
     for (int d = 0; d < DIM; d++) force[d] = (x[d] - q[d]) * a.k_rep;
@@ -98,17 +110,32 @@ __global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
     x[d] = a.p[d][i];
     np[d] = x[d];
   }
-  const int3 c = cell_of<F>(a.g, x[0], x[1], DIM == 3 ? x[2] : (F) 0);
-  auto body = [&](uint32_t j) {
-    F q[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) q[d] = a.p[d][j];
-    circle_visit<F, DIM>(a, x, q, np);
-  };
   if (BY_ID) {
+    const int3 c = cell_of<F>(a.g, x[0], x[1], DIM == 3 ? x[2] : (F) 0);
+    auto body = [&](uint32_t j) {
+      F q[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) q[d] = a.p[d][j];
+      circle_visit<F, DIM>(a, x, q, np);
+    };
     for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
   } else {
-    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+    // stencil rows narrowed to the cells within reach; candidates read
+    // through buffer resources at one 32-bit offset (a.n * sizeof(F) < 2^31,
+    // checked at launch)
+    ColBuf cb[3];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) cb[d] = col_buf(a.p[d], (uint32_t) a.n * (uint32_t) sizeof(F));
+    for_each_candidate_row(a.g, (double) x[0], (double) x[1], DIM == 3 ? (double) x[2] : 0.0, a.rc,
+                           [&](uint32_t j0, uint32_t j1) {
+                             for (uint32_t off = j0 * (uint32_t) sizeof(F);
+                                  off < j1 * (uint32_t) sizeof(F); off += (uint32_t) sizeof(F)) {
+                               F q[3] = {0, 0, 0};
+#pragma unroll
+                               for (int d = 0; d < DIM; d++) q[d] = col_load<F>(cb[d], off);
+                               circle_visit<F, DIM>(a, x, q, np);
+                             }
+                           });
   }
   circle_finish<F, DIM>(a, np, (uint32_t) i);
 }
@@ -199,18 +226,28 @@ __global__ __launch_bounds__(1024) void circle_step_c(CircleArgsC<DIM> a) {
   double x[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int d = 0; d < DIM; d++) x[d] = a.p[d][i];
-  const int3 c = cell_of<double>(a.g, x[0], x[1], x[2]);
   CircleAccC<DIM> s;
-  auto body = [&](uint32_t j) {
-    float o[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) o[d] = (float) (a.p[d][j] - x[d]);
-    s.visit(a, o);
-  };
   if (BY_ID) {
+    const int3 c = cell_of<double>(a.g, x[0], x[1], x[2]);
+    auto body = [&](uint32_t j) {
+      float o[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int d = 0; d < DIM; d++) o[d] = (float) (a.p[d][j] - x[d]);
+      s.visit(a, o);
+    };
     for_each_candidate_by_id(a.g, c.x, c.y, c.z, a.g.reach, a.id, body);
   } else {
-    for_each_candidate(a.g, c.x, c.y, c.z, a.g.reach, body);
+    ColBuf cb[3];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) cb[d] = col_buf(a.p[d], (uint32_t) a.n * 8u);
+    for_each_candidate_row(a.g, x[0], x[1], x[2], a.rc, [&](uint32_t j0, uint32_t j1) {
+      for (uint32_t off = j0 * 8u; off < j1 * 8u; off += 8u) {
+        float o[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) o[d] = (float) (col_load<double>(cb[d], off) - x[d]);
+        s.visit(a, o);
+      }
+    });
   }
   circle_finish_c<DIM>(a, s, x, (uint32_t) i);
 }
@@ -276,8 +313,10 @@ void launch_c(const StepLaunch &L) {
     else if (L.tile == 2) launch_tiles<M, 8, 8, 1>(L, a);
     else if (L.tile == 3) launch_tiles<M, 32, 8, 1>(L, a);
     else launch_tiles<M, 16, 8, 1>(L, a);
-  } else {
+  } else if ((size_t) L.n * 8u < ((size_t) 1 << 31)) {
     hipLaunchKernelGGL((circle_step_c<DIM, false>), nb, L.block, 0, L.stream, a);
+  } else {
+    hipLaunchKernelGGL((circle_step_c<DIM, true>), nb, L.block, 0, L.stream, a);
   }
 }
 
@@ -298,6 +337,9 @@ void launch_typed(const StepLaunch &L) {
   a.k_att = (F) L.params[P_K_ATT];
   a.W = (F) L.params[P_W];
   const unsigned nb = grid_blocks(L.n, L.block);
+  // buffer offsets of the untiled cell-order kernel are 32-bit (beyond
+  // 2^31 bytes of a column the index-order kernel runs instead)
+  const bool untiled_ok = (size_t) L.n * sizeof(F) < ((size_t) 1 << 31);
   if (L.order == 1) {
     hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
   } else if (L.tile != 0 && L.grid.reach <= 4) {
@@ -305,8 +347,10 @@ void launch_typed(const StepLaunch &L) {
     else if (L.tile == 2) launch_tiles<CirclePolicy<F, DIM>, 8, 8, 1>(L, a);
     else if (L.tile == 3) launch_tiles<CirclePolicy<F, DIM>, 32, 8, 1>(L, a);
     else launch_tiles<CirclePolicy<F, DIM>, 16, 8, 1>(L, a);
-  } else {
+  } else if (untiled_ok) {
     hipLaunchKernelGGL((circle_step<F, DIM, false>), nb, L.block, 0, L.stream, a);
+  } else {
+    hipLaunchKernelGGL((circle_step<F, DIM, true>), nb, L.block, 0, L.stream, a);
   }
 }
 
