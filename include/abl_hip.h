@@ -144,7 +144,10 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
  * from scratch with the split launches from then on),
  * "hip.timing" (1: time kernels with HIP events), "hip.seed" (key of
  * the counter-based generator random() uses inside steps), "hip.log_csv"
- * (file log_csv rows are appended to). */
+ * (file log_csv rows are appended to), "hip.defer_upload" ("1": from now on
+ * abl_hip_upload and abl_hip_init_agents keep the records on the host and
+ * make no device call, so another thread can build the initial state; "0":
+ * upload what was kept and return to direct uploads). */
 int abl_hip_set_config(abl_sim *sim, const char *key, const char *value);
 
 /* Agent types of the program, in declaration order. */

@@ -110,7 +110,8 @@ def _programs():
             out[f"golden_{case}_{prec}"] = (m, p, cfg)
     # models at their own defaults, for the GPU parity of the generated steps
     for m in ("circle", "circle3d", "boids2d", "boids2d_cfg", "circle3d_clustered",
-              "game_of_life", "predator_prey", "boids", "boids2d_flockers", "ants", "sugarscape"):
+              "game_of_life", "predator_prey", "boids", "boids2d_flockers", "ants", "sugarscape",
+              "keratinocyte"):
         out[f"model_{m}"] = (m, {}, {})
     return out
 

@@ -131,8 +131,8 @@ class Gen {
                              " has a type the hip backend cannot store");
         for (int k = 0; k < nc; k++) cols.push_back({(int) m, k, ct});
       }
-      if (cols.size() > 16)
-        throw BackendError("agent " + ag->name + " has more than 16 member components");
+      if (cols.size() > 32)
+        throw BackendError("agent " + ag->name + " has more than 32 member components");
       agent_cols[ag] = cols;
     }
   }
@@ -934,9 +934,30 @@ std::string Gen::device_program() {
       each_stmt(
           *s, [&](const Stmt &x) { if (x.for_kind == ForKind::Near) nl_stmt = &x; },
           [](const Expr &) {});
-    const Expr &r = *nl_stmt->e0->a[1];
+    const Expr *r = nl_stmt->e0->a[1].get();
+    // a local initialised with a constant expression and never assigned
+    // again stands for its initialiser, converted to its declared type
+    std::string cast;
+    if (r->kind == EK::Var && !global_of(*r)) {
+      const Stmt *decl = nullptr;
+      bool reassigned = false;
+      for (auto &st : f->body)
+        each_stmt(
+            *st,
+            [&](const Stmt &x) {
+              if (x.kind == SK::VarDecl && x.var.id == r->var.id) decl = &x;
+              if ((x.kind == SK::Assign || x.kind == SK::AssignOp) && x.e0 &&
+                  x.e0->kind == EK::Var && x.e0->var.id == r->var.id)
+                reassigned = true;
+            },
+            [](const Expr &) {});
+      if (decl && decl->e0 && !reassigned) {
+        cast = "(" + type_name(decl->type.resolved) + ") ";
+        r = decl->e0.get();
+      }
+    }
     bool constant = true;
-    each_expr(r, [&](const Expr &x) {
+    each_expr(*r, [&](const Expr &x) {
       if (x.kind == EK::Var && !global_of(x)) constant = false;
       if (x.kind == EK::Member || (x.kind == EK::Call && x.call != CallKind::Ctor &&
                                    x.call != CallKind::Builtin))
@@ -946,7 +967,7 @@ std::string Gen::device_program() {
       throw BackendError("The hip backend needs the near radius of " + f->name +
                          " to be a constant expression");
     o << "  static __host__ __device__ double radius_" << ident(f->name) << "() { return (double) ("
-      << expr(r) << "); }\n";
+      << (cast.empty() ? expr(*r) : cast + "(" + expr(*r) + ")") << "); }\n";
   }
   if (S.simulate && S.simulate->seq_decl) o << seq_step(*S.simulate->seq_decl);
   o << "};\n\n";

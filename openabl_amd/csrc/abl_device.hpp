@@ -20,7 +20,7 @@
 
 namespace abl {
 
-constexpr int kMaxCols = 16;
+constexpr int kMaxCols = 32;
 constexpr int kWave = 64;
 
 // Correctly rounded sqrtf.
