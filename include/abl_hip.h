@@ -95,6 +95,13 @@ void abl_rng_state(uint64_t out[2]);
 int abl_save_json(const void *agents, const abl_agent_info *info, const char *path,
                   int float_bytes);
 
+/* save(agents, info, path, SAVE_FLAME_XML) (for_gpu = 0) or SAVE_FLAMEGPU_XML
+ * (for_gpu = 1) (asset/c/libabl.c:126-213): the FLAME / FlameGPU state file
+ * of the agents, arguments as for abl_save_json.  ABL_ERR_ARG also when
+ * for_gpu and a position member is not a vector. */
+int abl_save_flame_xml(const void *agents, const abl_agent_info *info, const char *path,
+                       int float_bytes, int for_gpu);
+
 /* ---- simulations ------------------------------------------------------- */
 typedef struct abl_sim abl_sim;
 

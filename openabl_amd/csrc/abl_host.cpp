@@ -135,6 +135,79 @@ extern "C" int abl_save_json(const void *agents, const abl_agent_info *info, con
   return ABL_OK;
 }
 
+// ---- FLAME / FlameGPU XML ----------------------------------------------------
+// save(..., SAVE_FLAME_XML / SAVE_FLAMEGPU_XML) (asset/c/libabl.c:126-213):
+// <states>, <itno>0</itno>, one <xagent> per agent with its type's <name>
+// and one element per member (vectors as name_x, name_y[, name_z]); FlameGPU
+// writes the position member as x, y, z (z = 0.0 in 2-D).  Bools print as
+// 0/1 (the reference reads the bool's bytes as an int, which is 0/1 while
+// the struct padding after it is zero, as in every generated struct).
+
+namespace {
+
+void write_xml_member(FILE *f, const char *p, const abl_type_info *m, int fb, bool for_gpu) {
+  auto rd = [fb](const char *q, int k) -> double {
+    return fb == 8 ? ((const double *) q)[k] : (double) ((const float *) q)[k];
+  };
+  const char *n = m->name;
+  if (for_gpu && m->is_pos) {
+    if (m->type == ABL_TYPE_FLOAT2) {
+      fprintf(f, "<x>%f</x>\n<y>%f</y>\n<z>0.0</z>\n", rd(p, 0), rd(p, 1));
+    } else {
+      fprintf(f, "<x>%f</x>\n<y>%f</y>\n<z>%f</z>\n", rd(p, 0), rd(p, 1), rd(p, 2));
+    }
+    return;
+  }
+  switch (m->type) {
+    case ABL_TYPE_BOOL: fprintf(f, "<%s>%d</%s>\n", n, *(const bool *) p ? 1 : 0, n); break;
+    case ABL_TYPE_INT: fprintf(f, "<%s>%d</%s>\n", n, *(const int *) p, n); break;
+    case ABL_TYPE_FLOAT: fprintf(f, "<%s>%f</%s>\n", n, rd(p, 0), n); break;
+    case ABL_TYPE_FLOAT2:
+      fprintf(f, "<%s_x>%f</%s_x>\n<%s_y>%f</%s_y>\n", n, rd(p, 0), n, n, rd(p, 1), n);
+      break;
+    case ABL_TYPE_FLOAT3:
+      fprintf(f, "<%s_x>%f</%s_x>\n<%s_y>%f</%s_y>\n<%s_z>%f</%s_z>\n", n, rd(p, 0), n, n,
+              rd(p, 1), n, n, rd(p, 2), n);
+      break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+extern "C" int abl_save_flame_xml(const void *agents, const abl_agent_info *info, const char *path,
+                                  int float_bytes, int for_gpu) {
+  if (!agents || !info || !path || (float_bytes != 4 && float_bytes != 8)) return ABL_ERR_ARG;
+  for (const abl_agent_info *a = info; a->name; a++) {
+    for (const abl_type_info *m = a->info; m->type != ABL_TYPE_END; m++) {
+      if (for_gpu && m->is_pos && m->type != ABL_TYPE_FLOAT2 && m->type != ABL_TYPE_FLOAT3)
+        return ABL_ERR_ARG;  // FlameGPU positions are vectors
+    }
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) {
+    fprintf(stderr, "save(): Count not open \"%s\" for writing\n", path);
+    return ABL_ERR_ARG;
+  }
+  fputs("<states>\n<itno>0</itno>\n", f);
+  for (const abl_agent_info *a = info; a->name; a++) {
+    const abl_dyn_array *arr = (const abl_dyn_array *) ((const char *) agents + a->offset);
+    const abl_type_info *end = a->info;
+    while (end->type != ABL_TYPE_END) end++;
+    const size_t rec = end->offset;
+    for (size_t i = 0; i < arr->len; i++) {
+      const char *r = (const char *) arr->values + i * rec;
+      fprintf(f, "<xagent>\n<name>%s</name>\n", a->name);
+      for (const abl_type_info *m = a->info; m->type != ABL_TYPE_END; m++)
+        write_xml_member(f, r + m->offset, m, float_bytes, for_gpu != 0);
+      fputs("</xagent>\n", f);
+    }
+  }
+  fputs("</states>\n", f);
+  fclose(f);
+  return ABL_OK;
+}
+
 // ---- near-test thresholds --------------------------------------------------
 
 namespace abl {

@@ -36,6 +36,7 @@ _SIGNATURES = {
     "abl_rng_seed": ([C.c_uint64, C.c_uint64], None),
     "abl_rng_state": ([C.c_void_p], None),
     "abl_save_json": ([C.c_void_p, C.c_void_p, C.c_char_p, C.c_int], C.c_int),
+    "abl_save_flame_xml": ([C.c_void_p, C.c_void_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
     "abl_hip_program_count": ([], C.c_int),
     "abl_hip_program_name": ([C.c_int], C.c_char_p),
     "abl_hip_create": ([C.c_char_p, C.c_int, C.c_void_p], C.c_int),

@@ -12,6 +12,8 @@
  *   ref_probe dist2 <file>                 dist_float2 for "ax ay bx by" lines (hex in/out)
  *   ref_probe dist3 <file>                 dist_float3 for 6-value lines
  *   ref_probe save <path>                  save() of a fixed two-type record set
+ *   ref_probe save_flame <json> <xml>      the same records as JSON, then as FLAME XML
+ *   ref_probe save_flamegpu <json> <xml>   the same, FlameGPU XML
  */
 #ifdef ABL_RT_HEADER
 #include ABL_RT_HEADER
@@ -57,7 +59,7 @@ static const agent_info probe_info[] = {
   { NULL, 0, NULL }
 };
 
-static int probe_save(const char *path) {
+static void probe_records(void) {
   for (int i = 0; i < 5; i++) {
     ProbeA *a = DYN_ARRAY_PLACE(&probe.agents_ProbeA, ProbeA);
     a->pos = float2_create(random_float(-1000, 1000), random_float(0, 1));
@@ -70,7 +72,19 @@ static int probe_save(const char *path) {
     b->pos = float3_create(random_float(0, 1), random_float(0, 10), random_float(0, 100));
     b->vel = float3_create(-0.0000005, 0.0000005, -2.5);
   }
-  save(&probe, probe_info, path, SAVE_JSON);
+}
+
+static int probe_save(const char *path, save_type type) {
+  probe_records();
+  save(&probe, probe_info, path, type);
+  return 0;
+}
+
+/* the same records written as JSON and then as the given XML form */
+static int probe_save_both(const char *json_path, const char *xml_path, save_type type) {
+  probe_records();
+  save(&probe, probe_info, json_path, SAVE_JSON);
+  save(&probe, probe_info, xml_path, type);
   return 0;
 }
 
@@ -118,6 +132,9 @@ int main(int argc, char **argv) {
   }
   if (strcmp(mode, "dist2") == 0) return probe_dist(argv[2], 2);
   if (strcmp(mode, "dist3") == 0) return probe_dist(argv[2], 3);
-  if (strcmp(mode, "save") == 0) return probe_save(argv[2]);
+  if (strcmp(mode, "save") == 0) return probe_save(argv[2], SAVE_JSON);
+  if (strcmp(mode, "save_flame") == 0 && argc == 4) return probe_save_both(argv[2], argv[3], SAVE_FLAME_XML);
+  if (strcmp(mode, "save_flamegpu") == 0 && argc == 4)
+    return probe_save_both(argv[2], argv[3], SAVE_FLAMEGPU_XML);
   return 2;
 }

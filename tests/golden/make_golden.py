@@ -8,6 +8,8 @@ product's host runtime without the reference being present:
 
   rng_<prec>.json         random_float / random_int draw sequences (hex floats)
   save_<prec>.json        save() output of a fixed two-type record set
+  save_xml_<prec>.json    the same kind of records saved as JSON and as FLAME /
+                          FlameGPU XML
   dist_<prec>.json        dist_float2 / dist_float3 on pairs near rounding edges
   <model>_<prec>.json     final save() text of small runs (sha256 + agent count)
 
@@ -84,6 +86,19 @@ def main():
                 text = f.read()
         with open(os.path.join(HERE, f"save_{prec}.json"), "w") as f:
             json.dump({"text": text}, f)
+        # the same records as JSON and as FLAME / FlameGPU XML (save() with
+        # SAVE_FLAME_XML, SAVE_FLAMEGPU_XML)
+        xml = {}
+        for kind in ("flame", "flamegpu"):
+            with tempfile.TemporaryDirectory() as td:
+                pj, px = os.path.join(td, "s.json"), os.path.join(td, "s.xml")
+                run([probe, f"save_{kind}", pj, px])
+                with open(pj) as f:
+                    xml["json"] = f.read()
+                with open(px) as f:
+                    xml[kind] = f.read()
+        with open(os.path.join(HERE, f"save_xml_{prec}.json"), "w") as f:
+            json.dump(xml, f)
         dist = {}
         for dim in (2, 3):
             pairs = dist_pairs(dim, 7 + dim)
