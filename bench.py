@@ -254,8 +254,9 @@ def ensemble_members(k=256, seed=2026):
 
 def run_ensemble(args):
     """256 uneven boids2d members, one launch per kernel, graph-replayed
-    timesteps; beside it the same members run alone one after another (a
-    sample of them, extrapolated) on the plain per-run path."""
+    timesteps through neighbour lists (each member re-sorted and rebuilt on
+    its own schedule); beside it the same members run alone one after another
+    (a sample of them, extrapolated) on the plain per-run path."""
     from openabl_amd import Simulation, load_library
     from openabl_amd.ensemble import Ensemble, member_seed
     ms = ensemble_members()
@@ -294,7 +295,8 @@ def run_ensemble(args):
     return {"metric": "batched ensemble agent-steps/s (256 boids2d members)",
             "members": len(ms), "agents": total, "steps": args.ens_steps,
             "config": "N log-uniform [1k, 64k], box sqrt(N/0.065), weights U[0.5,2], "
-                      "cell_scale 2, 16x8 tiles, graph-replayed timesteps",
+                      "cell_scale 2, 16x8 tiles, neighbour lists (skin 0.05, rebuilds per "
+                      "member), graph-replayed timesteps",
             **res}
 
 

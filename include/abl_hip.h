@@ -251,7 +251,11 @@ int abl_hip_log_text(abl_sim *sim, char *buf, size_t capacity, size_t *len);
  * timestep is replayed as a captured HIP graph ("hip.graph", default on).
  * Member m's initialisation starts from abl_ens_member_seed(s0, s1, m); each
  * member's result is bit-identical to a run of it alone from that state with
- * the same "hip.cell_scale" and "hip.tile".  Programs: boids2d. */
+ * the same "hip.cell_scale", "hip.tile", "hip.nlist", "hip.skin" and
+ * "hip.list_cap".  Neighbour lists ("hip.nlist", default 1, when every member
+ * has the same near radius): one set over all members' agents, and a rebuild
+ * gate per member, so a member re-sorts and rebuilds its lists on exactly the
+ * timesteps a run of it alone does.  Programs: boids2d. */
 typedef struct abl_ensemble abl_ensemble;
 const char *abl_ens_last_error(void);
 int abl_ens_member_seed(uint64_t s0, uint64_t s1, int member, uint64_t out[2]);

@@ -12,10 +12,20 @@
 // neighbours visited, exactly as in a run of it alone, so each member's result
 // is bit-identical to that run.
 //
+// Neighbour lists ("hip.nlist", default on when the members share the near
+// radius): one set of lists over the concatenated slots, built after a sort
+// and walked while no agent of the member has moved more than skin / 2 since
+// (abl_tile.hpp).  Each member has its own rebuild gate, set by its own
+// agents, so it rebuilds on exactly the timesteps a run of it alone would.
+// The sort runs when some member is due; a member that is not keeps its
+// stored cell keys, so the sort leaves its agents where they are.  The sort
+// gathers into the other buffers and the list build copies the agents of the
+// rebuilt members back, so the step always reads the current buffers.
+//
 // With static populations every timestep issues the same kernels on the same
-// buffers (the grid gather and the step each flip the columns, so they return
-// to the same buffer each timestep): the timestep is captured once as a graph
-// and replayed.
+// buffers (without lists the grid gather and the step each flip the columns;
+// with lists the step flips them, and two timesteps return to the start): two
+// timesteps are captured once as a graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,17 +77,29 @@ __device__ __forceinline__ int member_of(const uint32_t *off, int members, uint3
   return lo;
 }
 
+// Every slot's cell key.  With a gate: only while *gate is set, and a member
+// whose own gate (mgate[m]) is clear keeps the key of the slot's cell at the
+// last sort (skey), so the sort leaves its agents in place.
 template <typename P>
 __global__ void k_ens_bin(const P *__restrict__ px, const P *__restrict__ py,
                           const uint32_t *__restrict__ off, const GridView *__restrict__ grids,
                           const uint32_t *__restrict__ cell_base, int members,
-                          uint32_t *__restrict__ key, uint32_t *__restrict__ count, int n) {
+                          uint32_t *__restrict__ key, uint32_t *__restrict__ count, int n,
+                          const uint32_t *__restrict__ gate = nullptr,
+                          const uint32_t *__restrict__ mgate = nullptr,
+                          const uint32_t *__restrict__ skey = nullptr) {
+  if (gate && *gate == 0u) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int m = member_of(off, members, (uint32_t) i);
-  const GridView &g = grids[m];
-  const int3 c = cell_of<P>(g, px[i], py[i], (P) 0);
-  const uint32_t k = cell_base[m] + (uint32_t) c.y * (uint32_t) g.n[0] + (uint32_t) c.x;
+  uint32_t k;
+  if (mgate && mgate[m] == 0u) {
+    k = skey[i];
+  } else {
+    const GridView &g = grids[m];
+    const int3 c = cell_of<P>(g, px[i], py[i], (P) 0);
+    k = cell_base[m] + (uint32_t) c.y * (uint32_t) g.n[0] + (uint32_t) c.x;
+  }
   key[i] = k;
   atomicAdd(&count[k], 1u);
 }
@@ -121,6 +143,21 @@ struct abl_ensemble {
   int cell_scale = 2, tile = 1, block = 256;
   bool use_graph = true;
   bool ready = false;
+  uint64_t timestep = 0;       // timesteps issued (the lists' gates go by its parity)
+  // neighbour lists
+  bool nlist = true;           // "hip.nlist": lists when the members share the near radius
+  double skin = kDefaultSkin;  // "hip.skin": list skin as a fraction of the near radius
+  int list_cap = 0;            // "hip.list_cap": entries per list (0: sized from the data)
+  bool lists = false;          // in use (decided by abl_ens_init)
+  double r = 0.0, rl = 0.0, cell = 0.0;
+  int nl_cap = 0;
+  uint16_t *nl_idx = nullptr, *nl_len = nullptr;
+  uint32_t *nl_perm = nullptr;
+  uint4 *nl_prow0 = nullptr;
+  double *nl_pos0 = nullptr;
+  uint32_t *nl_gate = nullptr;   // [0..1] some member due, by parity; [2] longest list
+  uint32_t *nl_mgate = nullptr;  // [2][members] member due, by parity
+  uint32_t *skey = nullptr;      // each slot's cell key at the last sort
   hipStream_t stream = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -201,6 +238,17 @@ extern "C" int abl_ens_set_config(abl_ensemble *e, const char *key, const char *
     e->block = b;
   } else if (k == "hip.graph") {
     e->use_graph = v == "1" || v == "true";
+  } else if (k == "hip.nlist") {
+    if (v != "0" && v != "1") return ens_fail(ABL_ERR_ARG, "hip.nlist must be 0 or 1");
+    e->nlist = v == "1";
+  } else if (k == "hip.skin") {
+    const double f = std::atof(value);
+    if (!(f > 0.0 && f <= 1.0)) return ens_fail(ABL_ERR_ARG, "hip.skin must be in (0, 1]");
+    e->skin = f;
+  } else if (k == "hip.list_cap") {
+    const int c = std::atoi(value);
+    if (c < 0 || c > 65528) return ens_fail(ABL_ERR_ARG, "hip.list_cap must be 0..65528");
+    e->list_cap = c;
   } else {
     return ens_fail(ABL_ERR_ARG, "unknown config key " + k);
   }
@@ -219,10 +267,15 @@ static void free_all(abl_ensemble *e) {
     e->d_args[b] = nullptr;
   }
   void *p[] = {e->d_off, e->d_cell_base, e->d_tile_base, e->d_grids, e->key, e->count,
-               e->start, e->bsum, e->tmp_idx, e->tmp_id};
+               e->start, e->bsum, e->tmp_idx, e->tmp_id, e->nl_idx, e->nl_len, e->nl_perm,
+               e->nl_prow0, e->nl_pos0, e->nl_gate, e->nl_mgate, e->skey};
   for (void *q : p) {
     if (q) (void) hipFree(q);
   }
+  e->nl_idx = e->nl_len = nullptr;
+  e->nl_perm = e->nl_gate = e->nl_mgate = e->skey = nullptr;
+  e->nl_prow0 = nullptr;
+  e->nl_pos0 = nullptr;
   e->d_off = e->d_cell_base = e->d_tile_base = e->key = e->count = e->start = e->bsum = nullptr;
   e->tmp_idx = nullptr;
   e->tmp_id = nullptr;
@@ -231,6 +284,140 @@ static void free_all(abl_ensemble *e) {
   if (e->graph) (void) hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
+}
+
+// The list arguments of the current timestep (as a run alone forms them).
+static NListArgs ens_list_args(const abl_ensemble *e) {
+  NListArgs nl{};
+  nl.idx = e->nl_idx;
+  nl.len = e->nl_len;
+  nl.perm = e->nl_perm;
+  nl.prow0 = e->nl_prow0;
+  nl.pos0 = e->nl_pos0;
+  nl.gate = e->nl_gate;
+  nl.stats = e->nl_gate + 2;
+  nl.n = (int) e->n;
+  nl.cap = e->nl_cap;
+  nl.stride = (int) e->n;
+  nl.parity = (int) (e->timestep & 1u);
+  nl.rc_list = (float) (e->rl / e->cell * (1.0 + 1.0 / 1048576.0));
+  const double rw = e->rl * (1.0 + 1e-6) + 1e-5 * e->r;
+  nl.r2_list = rw * rw;
+  const double h = 0.5 * (e->rl - e->r);
+  nl.half_skin2 = h * h;
+  nl.ncells = (int) e->ncells;
+  return nl;
+}
+
+// The gated rebuild of the current timestep: the sort of every slot into the
+// other buffers (members not due keep their places), then the lists of the
+// members due, which copy their agents back to the current buffers.
+static void issue_rebuild(abl_ensemble *e) {
+  const int n = (int) e->n;
+  const int M = (int) e->mem.size();
+  hipStream_t st = e->stream;
+  const int b = e->cur;
+  const uint32_t par = (uint32_t) (e->timestep & 1u);
+  const uint32_t *gate = e->nl_gate + par;
+  const uint32_t *mg = e->nl_mgate + (size_t) par * (size_t) M;
+  if (e->elem == 8) {
+    hipLaunchKernelGGL(k_ens_bin<double>, grid_blocks(n, 256), 256, 0, st,
+                       (const double *) e->col[b][0], (const double *) e->col[b][1], e->d_off,
+                       e->d_grids, e->d_cell_base, M, e->key, e->count, n, gate, mg, e->skey);
+  } else {
+    hipLaunchKernelGGL(k_ens_bin<float>, grid_blocks(n, 256), 256, 0, st,
+                       (const float *) e->col[b][0], (const float *) e->col[b][1], e->d_off,
+                       e->d_grids, e->d_cell_base, M, e->key, e->count, n, gate, mg, e->skey);
+  }
+  const int nc = (int) e->ncells;
+  const int ntiles = (nc + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_scan_tiles<uint32_t, false>), ntiles, kScanBlock, 0, st, e->count, e->start,
+                     e->bsum, nc, gate);
+  hipLaunchKernelGGL(k_scan_totals, 1, 1024, 0, st, e->bsum, ntiles, e->start, nc, gate);
+  hipLaunchKernelGGL(k_scan_add, grid_blocks(nc, 256), 256, 0, st, e->start, e->bsum, nc, gate);
+  hipLaunchKernelGGL(k_scatter, grid_blocks(n, 256), 256, 0, st, e->key, e->count, e->start,
+                     e->id[e->id_cur], e->tmp_idx, e->tmp_id, n, gate);
+  GatherCols gc{};
+  gc.ncols = e->ncols;
+  for (int c = 0; c < e->ncols; c++) {
+    gc.src[c] = e->col[b][c];
+    gc.dst[c] = e->col[1 - b][c];
+    gc.elem[c] = e->elem;
+  }
+  hipLaunchKernelGGL(k_rank_gather, grid_blocks(n, 256), 256, 0, st, e->tmp_idx, e->tmp_id,
+                     e->key, e->start, gc, e->id[1 - e->id_cur], n, e->skey, nullptr, gate);
+  NListArgs nl = ens_list_args(e);
+  nl.gated = 1;
+  nl.copy.ncols = e->ncols;
+  for (int c = 0; c < e->ncols; c++) {
+    nl.copy.src[c] = e->col[1 - b][c];
+    nl.copy.dst[c] = e->col[b][c];
+    nl.copy.elem[c] = e->elem;
+  }
+  nl.id_src = e->id[1 - e->id_cur];
+  nl.id_dst = e->id[e->id_cur];
+  nl.skey_src = e->skey;
+  nl.skey_dst = e->skey;
+  // the build reads the sorted buffers: the arguments whose input is 1 - b
+  e->hooks->launch_multi_list(e->precision, e->d_args[b], e->d_tile_base, M, e->total_tiles,
+                              e->reach, e->occupancy, e->tile, e->block, st, nl, e->nl_mgate, 1);
+}
+
+// Every member due a rebuild at this parity (p) and none at the other.
+static int set_gates(abl_ensemble *e, uint32_t p) {
+  const size_t M = e->mem.size();
+  std::vector<uint32_t> g(16, 0u), mg(2 * M, 0u);
+  g[p] = 1u;
+  for (size_t m = 0; m < M; m++) mg[p * M + m] = 1u;
+  ENS_TRY(hipMemcpy(e->nl_gate, g.data(), 16 * 4, hipMemcpyHostToDevice));
+  ENS_TRY(hipMemcpy(e->nl_mgate, mg.data(), 2 * M * 4, hipMemcpyHostToDevice));
+  return ABL_OK;
+}
+
+// List buffers, and every member's first sort and lists (as a run alone
+// enters list mode: capacity "hip.list_cap", or from the expected population
+// of the list radius, grown once to the longest list, at most four times the
+// estimate).
+static int init_lists(abl_ensemble *e) {
+  const size_t N = e->n;
+  const size_t M = e->mem.size();
+  int want = 16;
+  for (const Member &mb : e->mem) {
+    const double env = (double) mb.g.n[0] * (double) mb.g.n[1] * mb.cell * mb.cell;
+    const double expect = env > 0 ? M_PI * e->rl * e->rl * (double) mb.n / env : 0.0;
+    want = std::max(want, (int) (1.5 * expect) + 16);
+  }
+  if (e->list_cap) want = e->list_cap;
+  int cap = std::min((want + 7) & ~7, 65528);
+  ENS_TRY(hipMalloc(&e->nl_len, N * 2));
+  ENS_TRY(hipMalloc(&e->nl_perm, N * 4));
+  ENS_TRY(hipMalloc(&e->nl_prow0, N * 16));
+  ENS_TRY(hipMalloc(&e->nl_pos0, 2 * N * 8));
+  ENS_TRY(hipMalloc(&e->nl_gate, 16 * 4));
+  ENS_TRY(hipMalloc(&e->nl_mgate, 2 * M * 4));
+  ENS_TRY(hipMalloc(&e->skey, N * 4));
+  for (int attempt = 0;; attempt++) {
+    if (e->nl_idx) ENS_TRY(hipFree(e->nl_idx));
+    e->nl_idx = nullptr;
+    ENS_TRY(hipMalloc(&e->nl_idx, (size_t) cap * N * 2));
+    e->nl_cap = cap;
+    int rc = set_gates(e, 0u);
+    if (rc) return rc;
+    issue_rebuild(e);
+    ENS_TRY(hipGetLastError());
+    ENS_TRY(hipStreamSynchronize(e->stream));
+    uint32_t longest = 0;
+    ENS_TRY(hipMemcpy(&longest, e->nl_gate + 2, 4, hipMemcpyDeviceToHost));
+    const int grow = std::min((int) (1.25 * longest) + 16, 4 * want);
+    if (attempt > 0 || (int) longest <= cap || grow <= cap || cap >= 65528 || e->list_cap) break;
+    cap = std::min((grow + 7) & ~7, 65528);
+  }
+  // the lists are those of the initial state: no member is due at the first
+  // timestep (an overflowed list sets its member's gate at that step)
+  std::vector<uint32_t> zero(std::max<size_t>(16, 2 * M), 0u);
+  ENS_TRY(hipMemcpy(e->nl_gate, zero.data(), 16 * 4, hipMemcpyHostToDevice));
+  ENS_TRY(hipMemcpy(e->nl_mgate, zero.data(), 2 * M * 4, hipMemcpyHostToDevice));
+  return ABL_OK;
 }
 
 // Initial state of every member (main()'s agent creation from the member's
@@ -246,6 +433,42 @@ extern "C" int abl_ens_init(abl_ensemble *e, uint64_t s0, uint64_t s1) {
   e->hooks->tile_shape(e->tile, &tx, &ty, &tz);
   e->reach = 1;
   double occ_max = 0.0;
+  // the near radius of member m (under use_float the parameter is a float
+  // global, as in a run alone)
+  auto radius_of = [&](const Member &mb) {
+    const StepDesc &sd = e->prog->steps[0];
+    double r = 0.0;
+    for (size_t p = 0; p < e->prog->params.size(); p++) {
+      if (std::strcmp(e->prog->params[p].name, sd.radius_param) == 0) {
+        double v = mb.params[p];
+        if (e->precision != ABL_PREC_F64) v = (double) (float) v;
+        r = sd.radius_scale * v;
+      }
+    }
+    return r;
+  };
+  // neighbour lists when every member has the same near radius and cell size
+  // and the list radius needs no more than four cells of reach (as for a run
+  // alone); their cells are widened by 1 + skin, as a run alone's are
+  e->lists = e->nlist && e->hooks->launch_multi_list && M > 0;
+  double gran0 = 0.0;
+  for (int m = 0; m < M && e->lists; m++) {
+    double lo[3], hi[3], gran;
+    e->prog->env(e->mem[(size_t) m].params.data(), lo, hi, &gran);
+    const double r = radius_of(e->mem[(size_t) m]);
+    if (m == 0) {
+      gran0 = gran;
+      e->r = r;
+    } else if (gran != gran0 || r != e->r) {
+      e->lists = false;
+    }
+  }
+  if (e->lists) {
+    e->rl = e->r * (1.0 + e->skin);
+    const double cell = gran0 / (double) e->cell_scale * (1.0 + 1.0 / 1048576.0) * (1.0 + e->skin);
+    if (!(e->r > 0.0) || std::ceil(e->rl / cell * (1.0 + 1.0 / 2097152.0)) > 4.0) e->lists = false;
+  }
+  const double inflate = e->lists ? 1.0 + e->skin : 1.0;
   for (int m = 0; m < M; m++) {
     Member &mb = e->mem[(size_t) m];
     uint64_t st[2];
@@ -261,7 +484,7 @@ extern "C" int abl_ens_init(abl_ensemble *e, uint64_t s0, uint64_t s1) {
     host.insert(host.end(), recs.begin(), recs.end());
     double lo[3], hi[3], gran;
     e->prog->env(mb.params.data(), lo, hi, &gran);
-    const double cell = gran / (double) e->cell_scale * (1.0 + 1.0 / 1048576.0);
+    const double cell = gran / (double) e->cell_scale * (1.0 + 1.0 / 1048576.0) * inflate;
     GridView g{};
     g.dim = 2;
     for (int d = 0; d < 3; d++) {
@@ -269,16 +492,10 @@ extern "C" int abl_ens_init(abl_ensemble *e, uint64_t s0, uint64_t s1) {
       g.n[d] = d < 2 ? std::max(1, (int) std::ceil((hi[d] - lo[d]) / cell)) : 1;
     }
     g.inv_cell = 1.0 / cell;
-    const StepDesc &sd = e->prog->steps[0];
-    double r = 0.0;
-    for (size_t p = 0; p < e->prog->params.size(); p++) {
-      if (std::strcmp(e->prog->params[p].name, sd.radius_param) == 0) {
-        double v = mb.params[p];
-        if (e->precision != ABL_PREC_F64) v = (double) (float) v;
-        r = sd.radius_scale * v;
-      }
-    }
-    g.reach = std::max(1, (int) std::ceil(r / cell * (1.0 + 1.0 / 2097152.0)));
+    const double r = radius_of(mb);
+    // the stencil reaches the list radius in list mode
+    const double reach_r = e->lists ? e->rl : r;
+    g.reach = std::max(1, (int) std::ceil(reach_r / cell * (1.0 + 1.0 / 2097152.0)));
     e->reach = std::max(e->reach, g.reach);
     mb.g = g;
     mb.cell = cell;
@@ -292,6 +509,7 @@ extern "C" int abl_ens_init(abl_ensemble *e, uint64_t s0, uint64_t s1) {
     occ_max = std::max(occ_max, (double) cnt / ((double) g.n[0] * g.n[1]));
   }
   for (Member &mb : e->mem) mb.g.reach = e->reach;  // one kernel instance for all members
+  if (e->lists) e->cell = e->mem[0].cell;
   e->n = total;
   e->ncells = cells;
   e->total_tiles = tiles;
@@ -363,18 +581,40 @@ extern "C" int abl_ens_init(abl_ensemble *e, uint64_t s0, uint64_t s1) {
     ENS_TRY(hipMalloc(&e->d_args[b], args.size()));
     ENS_TRY(hipMemcpy(e->d_args[b], args.data(), args.size(), hipMemcpyHostToDevice));
   }
+  e->timestep = 0;
+  if (e->lists && total > 0) {
+    int rc = init_lists(e);
+    if (rc) return rc;
+  }
   ENS_TRY(hipStreamSynchronize(e->stream));
   e->ready = true;
   return ABL_OK;
 }
 
 // One timestep: rebuild every member's grid (one sort over all cells), then
-// the program's tiled step over all members' tiles.
+// the program's tiled step over all members' tiles.  With lists: the gated
+// rebuild, then the step through the lists from the current buffers into the
+// other ones.
 static int issue_step(abl_ensemble *e) {
   const int n = (int) e->n;
   const int M = (int) e->mem.size();
   hipStream_t st = e->stream;
   const int b = e->cur;
+  if (e->lists) {
+    if (n > 0) {
+      issue_rebuild(e);
+      NListArgs nl = ens_list_args(e);
+      // the step reads buffer b: the arguments whose input is 1 - (1 - b)
+      e->hooks->launch_multi_list(e->precision, e->d_args[1 - b], e->d_tile_base, M,
+                                  e->total_tiles, e->reach, e->occupancy, e->tile, e->block, st,
+                                  nl, e->nl_mgate, 2);
+      e->cur ^= 1;
+    }
+    e->timestep += 1;
+    ENS_TRY(hipGetLastError());
+    return ABL_OK;
+  }
+  e->timestep += 1;
   if (n > 0) {
     if (e->elem == 8) {
       hipLaunchKernelGGL(k_ens_bin<double>, grid_blocks(n, 256), 256, 0, st,
@@ -419,21 +659,28 @@ extern "C" int abl_ens_simulate(abl_ensemble *e, int steps) {
   ENS_TRY(hipEventRecord(e->ev0, e->stream));
   int t = 0;
   if (e->use_graph && steps > 0) {
+    // the id buffer (without lists) or the column buffers and the gates'
+    // parity (with lists) alternate per step: two timesteps, captured from
+    // an even start, end where they start
+    auto even = [&] { return e->lists ? (e->timestep & 1u) == 0 : (e->id_cur & 1) == 0; };
     if (!e->exec) {
-      // the id buffer alternates per step: capture two timesteps so the graph
-      // ends where it starts
-      if ((e->id_cur & 1) == 0 && steps >= 2) {
+      if (even() && steps >= 2) {
+        const uint64_t t0 = e->timestep;
         ENS_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         int rc = issue_step(e);
         if (!rc) rc = issue_step(e);
         hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+        e->timestep = t0;  // captured, not run
         if (rc) return rc;
         ENS_TRY(ce);
         ENS_TRY(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
       }
     }
-    if (e->exec && (e->id_cur & 1) == 0) {
-      for (; t + 2 <= steps; t += 2) ENS_TRY(hipGraphLaunch(e->exec, e->stream));
+    if (e->exec && even()) {
+      for (; t + 2 <= steps; t += 2) {
+        ENS_TRY(hipGraphLaunch(e->exec, e->stream));
+        e->timestep += 2;
+      }
     }
   }
   for (; t < steps; t++) {

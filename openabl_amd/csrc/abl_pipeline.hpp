@@ -21,10 +21,13 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 
 // Per tile: exclusive scan of count into start, tile total into bsum.
 // INV scans 1 - count (the survivors of a removeCurrent() flag array).
+// With a gate the sort kernels here run only while *gate is set.
 template <typename T, bool INV>
 static __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const T *__restrict__ count,
                                                           uint32_t *__restrict__ start,
-                                                          uint32_t *__restrict__ bsum, int ncells) {
+                                                          uint32_t *__restrict__ bsum, int ncells,
+                                                          const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t lds[kWave + 1];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
@@ -47,7 +50,9 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const T *__res
 
 // Exclusive scan of the tile totals (one block, any number of tiles).
 static __global__ __launch_bounds__(1024) void k_scan_totals(uint32_t *__restrict__ bsum, int ntiles,
-                                                     uint32_t *__restrict__ start, int ncells) {
+                                                     uint32_t *__restrict__ start, int ncells,
+                                                     const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t lds[kWave + 1];
   uint32_t carry = 0;
   for (int base = 0; base < ntiles; base += blockDim.x) {
@@ -62,14 +67,17 @@ static __global__ __launch_bounds__(1024) void k_scan_totals(uint32_t *__restric
 }
 
 static __global__ void k_scan_add(uint32_t *__restrict__ start, const uint32_t *__restrict__ bsum,
-                           int ncells) {
+                           int ncells, const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncells) start[c] += bsum[c / kScanTile];
 }
 
 static __global__ void k_scatter(const uint32_t *__restrict__ key, uint32_t *__restrict__ count,
                           const uint32_t *__restrict__ start, const int32_t *__restrict__ id,
-                          uint32_t *__restrict__ tmp_idx, int32_t *__restrict__ tmp_id, int n) {
+                          uint32_t *__restrict__ tmp_idx, int32_t *__restrict__ tmp_id, int n,
+                          const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = key[i];
@@ -86,7 +94,9 @@ static __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const
                               const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
                               GatherCols g, int32_t *__restrict__ id_out, int n,
                               uint32_t *__restrict__ skey_out = nullptr,
-                              uint32_t *__restrict__ count_out = nullptr) {
+                              uint32_t *__restrict__ count_out = nullptr,
+                              const uint32_t *__restrict__ gate = nullptr) {
+  if (gate && *gate == 0u) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t src = tmp_idx[s];

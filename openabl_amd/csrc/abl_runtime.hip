@@ -199,9 +199,6 @@ struct AgentStore {
 
 }  // namespace
 
-// Default list skin, as a fraction of the near radius.
-constexpr double kDefaultSkin = 0.05;
-
 struct abl_sim {
   const ProgramDesc *prog = nullptr;
   int precision = ABL_PREC_F64;

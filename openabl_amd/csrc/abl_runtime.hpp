@@ -45,6 +45,10 @@ struct ParamDesc {
 struct StepLaunch;
 struct ColSet;
 struct DeltaArgs;
+struct NListArgs;
+
+// Default neighbour-list skin, as a fraction of the near radius.
+constexpr double kDefaultSkin = 0.05;
 
 // Hooks a single-type, single-step program provides so many independent runs
 // of it can be batched into one launch per kernel (abl_ensemble.hip).
@@ -63,6 +67,13 @@ struct EnsembleHooks {
   // main()'s agent records for these parameters (generator already seeded)
   int (*host_records)(const double *params, int precision, std::vector<char> &out,
                       size_t *count);
+  // the gated list build (mode 1) or the step through the lists (mode 2)
+  // over every member's tiles, with the members' own rebuild gates (null:
+  // the program has no list form)
+  void (*launch_multi_list)(int precision, const void *d_args, const uint32_t *d_tile_base,
+                            int members, unsigned total_tiles, int reach, float occupancy,
+                            int tile, int block, hipStream_t stream, const NListArgs &nl,
+                            uint32_t *mgate, int mode);
 };
 
 struct StepDesc {

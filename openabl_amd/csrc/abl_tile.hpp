@@ -1031,17 +1031,17 @@ __global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListAr
 // agent, list length and first list row are read together with the tile's
 // records (one round of global reads).  Which lane handles an agent never
 // changes the agent's results.
+//
+// Returns whether one of the thread's agents moved more than skin / 2 since
+// the build or walked the stencil (a rebuild is due).
 template <class M, int TX, int TY, int TZ, int RM>
-__global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list(
-    typename M::Args a, NListArgs nl) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ bool step_list_tile(const typename M::Args &a, const NListArgs &nl,
+                                               unsigned tile, unsigned char *smem) {
   using F = typename M::F;
   using Ctx = TileCtx<M, TX, TY, TZ, RM>;
   using TT = typename Ctx::TT;
   constexpr int DIM = M::DIM;
   constexpr int NS = M::NS;
-  const unsigned tile = xcd_block(blockIdx.x, gridDim.x);
-  if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
   // own agent k (< 256) of the row-group staging: its slot and LDS record
   __shared__ uint2 ownmap[256];
   // the agent of position k of the tile's order (own index << 16 | list
@@ -1138,7 +1138,71 @@ __global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_l
     // !(d2 <= bound) also catches NaN positions
     if (!(d2 <= nl.half_skin2)) stray = true;
   }
-  if (stray) nl.gate[nl.parity ^ 1] = 1u;
+  return stray;
+}
+
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list(
+    typename M::Args a, NListArgs nl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (blockIdx.x == 0 && threadIdx.x == 0) nl.gate[nl.parity] = 0u;
+  if (step_list_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(blockIdx.x, gridDim.x), smem))
+    nl.gate[nl.parity ^ 1] = 1u;
+}
+
+// ---- ensemble forms of the list build and the list step ------------------------
+// The tiles of many members in one launch (member lookup as in
+// tiled_step_multi) over one set of lists indexed by the members'
+// concatenated slots.  Besides the ensemble's gate (nl.gate: some member is
+// due a rebuild) every member has its own, by timestep parity: mgate[p *
+// members + m].  A member rebuilds exactly when a run of it alone would: when
+// one of its own agents strayed or overflowed.
+
+__device__ __forceinline__ int tile_member(const uint32_t *__restrict__ tile_base, int members,
+                                           unsigned t) {
+  int lo = 0, hi = members - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_base[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The lists of the tiles of members due a rebuild, after the gated sort
+// (records in the sorted buffers args point at; copied back as in
+// tiled_list_build).
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256) void tiled_list_build_multi(const typename M::Args *__restrict__ args,
+                                                              const uint32_t *__restrict__ tile_base,
+                                                              int members, NListArgs nl,
+                                                              const uint32_t *__restrict__ mgate) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (nl.gate[nl.parity] == 0u) return;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  const int m = tile_member(tile_base, members, t);
+  if (mgate[nl.parity * members + m] == 0u) return;
+  const uint32_t longest = list_build_tile<M, TX, TY, TZ, RM>(args[m], nl, t - tile_base[m], smem);
+  list_longest(nl, longest);
+}
+
+// One timestep of every member through the lists; clears this parity's gates
+// and sets the next parity's for the members that strayed.
+template <class M, int TX, int TY, int TZ, int RM>
+__global__ __launch_bounds__(256, policy_list_waves<M>::value) void tiled_step_list_multi(
+    const typename M::Args *__restrict__ args, const uint32_t *__restrict__ tile_base,
+    int members, NListArgs nl, uint32_t *__restrict__ mgate) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  const int m = tile_member(tile_base, members, t);
+  const unsigned first = tile_base[m];
+  if (threadIdx.x == 0) {
+    if (blockIdx.x == 0) nl.gate[nl.parity] = 0u;
+    if (t == first) mgate[nl.parity * members + m] = 0u;
+  }
+  if (step_list_tile<M, TX, TY, TZ, RM>(args[m], nl, t - first, smem)) {
+    nl.gate[nl.parity ^ 1] = 1u;
+    mgate[(nl.parity ^ 1) * members + m] = 1u;
+  }
 }
 
 // Host side: LDS record capacity (`slack` times the expected halo population

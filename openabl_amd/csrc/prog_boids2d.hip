@@ -534,8 +534,49 @@ void ens_launch_multi(int precision, const void *args, const uint32_t *tb, int m
   else ens_launch_t<float>(args, tb, m, total, reach, occ, tile, block, st);
 }
 
+template <typename F, int TX, int TY, int RM>
+void ens_launch_list(const void *args, const uint32_t *tile_base, int members, unsigned total,
+                     float occupancy, int block, hipStream_t st, const NListArgs &nl,
+                     uint32_t *mgate, int mode) {
+  int cap = 0;
+  size_t smem = 0;
+  ens_cap<F, TX, TY, RM>(occupancy, &cap, &smem);
+  const int threads = block > 256 ? 256 : block;
+  if (mode == 1)
+    hipLaunchKernelGGL((tiled_list_build_multi<BoidPolicy<F>, TX, TY, 1, RM>), total, threads, smem,
+                       st, (const BoidArgs<F> *) args, tile_base, members, nl, mgate);
+  else
+    hipLaunchKernelGGL((tiled_step_list_multi<BoidPolicy<F>, TX, TY, 1, RM>), total, threads, smem,
+                       st, (const BoidArgs<F> *) args, tile_base, members, nl, mgate);
+}
+
+template <typename F, int TX, int TY>
+void ens_launch_list_r(const void *args, const uint32_t *tb, int m, unsigned total, int reach,
+                       float occ, int block, hipStream_t st, const NListArgs &nl, uint32_t *mg,
+                       int mode) {
+  if (reach <= 1) ens_launch_list<F, TX, TY, 1>(args, tb, m, total, occ, block, st, nl, mg, mode);
+  else if (reach == 2) ens_launch_list<F, TX, TY, 2>(args, tb, m, total, occ, block, st, nl, mg, mode);
+  else ens_launch_list<F, TX, TY, 4>(args, tb, m, total, occ, block, st, nl, mg, mode);
+}
+
+template <typename F>
+void ens_launch_list_t(const void *args, const uint32_t *tb, int m, unsigned total, int reach,
+                       float occ, int tile, int block, hipStream_t st, const NListArgs &nl,
+                       uint32_t *mg, int mode) {
+  if (tile == 2) ens_launch_list_r<F, 8, 8>(args, tb, m, total, reach, occ, block, st, nl, mg, mode);
+  else if (tile == 3) ens_launch_list_r<F, 32, 8>(args, tb, m, total, reach, occ, block, st, nl, mg, mode);
+  else ens_launch_list_r<F, 16, 8>(args, tb, m, total, reach, occ, block, st, nl, mg, mode);
+}
+
+void ens_launch_multi_list(int precision, const void *args, const uint32_t *tb, int m,
+                           unsigned total, int reach, float occ, int tile, int block,
+                           hipStream_t st, const NListArgs &nl, uint32_t *mgate, int mode) {
+  if (precision == 0) ens_launch_list_t<double>(args, tb, m, total, reach, occ, tile, block, st, nl, mgate, mode);
+  else ens_launch_list_t<float>(args, tb, m, total, reach, occ, tile, block, st, nl, mgate, mode);
+}
+
 const EnsembleHooks ensemble_hooks = {ens_args_size, ens_tile_shape, ens_fill_args,
-                                      ens_launch_multi, host_records};
+                                      ens_launch_multi, host_records, ens_launch_multi_list};
 
 void launch_update_boid(const StepLaunch &L) {
   if (L.precision == ABL_PREC_F32C) launch_c(L);

@@ -1,8 +1,9 @@
 """Python binding of batched ensembles (abl_ens_*, include/abl_hip.h).
 
 Many independent runs of one program -- uneven agent counts, their own
-parameters -- advanced together: one grid sort and one tiled step launch per
-timestep for all members, replayed as a captured HIP graph.
+parameters -- advanced together: one tiled step launch per timestep for all
+members (through neighbour lists, re-sorted and rebuilt per member on the
+timesteps a run of it alone would), replayed as a captured HIP graph.
 """
 from __future__ import annotations
 

@@ -189,13 +189,17 @@ def ensemble_members(k=256, seed=11):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_ensemble_256_members_500_steps(prec):
+@pytest.mark.parametrize("lists", [1, 0])
+def test_ensemble_256_members_500_steps(prec, lists):
     from openabl_amd import Simulation, load_library
     from openabl_amd.ensemble import Ensemble, member_seed
     ms = ensemble_members()
-    print(f"case ensemble 256 members {prec}: {sum(m['num_agents'] for m in ms)} agents",
-          flush=True)
-    cfg = {"hip.cell_scale": 2, "hip.tile": 1}
+    print(f"case ensemble 256 members {prec} lists={lists}: "
+          f"{sum(m['num_agents'] for m in ms)} agents", flush=True)
+    cfg = {"hip.cell_scale": 2, "hip.tile": 1, "hip.nlist": lists}
+    if lists:
+        # fixed, so a list outgrows it in the ensemble exactly when it does alone
+        cfg["hip.list_cap"] = 128
     sample = [0, 97, int(np.argmax([m["num_agents"] for m in ms])), 255]
     ens = Ensemble("boids2d", prec, ms, config=cfg)
     ens.init(0x1234, 0x5678)
@@ -204,7 +208,7 @@ def test_ensemble_256_members_500_steps(prec):
     dt = dtype_of("boids2d", prec)
     for m in sample:
         load_library().abl_rng_seed(*member_seed(0x1234, 0x5678, m))
-        sim = Simulation("boids2d", prec, params=ms[m], config=dict(cfg, **{"hip.nlist": 0}))
+        sim = Simulation("boids2d", prec, params=ms[m], config=cfg)
         sim.init_agents()
         sim.simulate(500)
         assert field_bytes(ens.download(m)) == field_bytes(sim.download(dt)), m
