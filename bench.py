@@ -31,7 +31,11 @@ The JSON line carries
                    Poisson stream of mixed runs (1k-1M agents, 100-1000 steps);
   ensemble         256 uneven boids2d members batched, against members alone;
   precision_circle3d  1M clustered circle3d: FP64, FP32-compensated and FP32
-                   agent-steps/s and the per-step error against FP64.
+                   agent-steps/s and the per-step error against FP64;
+  launch_config    the launch configuration of the runs: CONFIG below, keys
+                   overridden with --set hip.KEY=VALUE, or chosen by
+                   --autotune (openabl_amd.autotune on this workload's initial
+                   state; its search is reported under "autotune").
 """
 from __future__ import annotations
 
@@ -64,6 +68,39 @@ PARAMS = dict(num_agents=N_AGENTS, agent_density=DENSITY, interaction_radius=10.
 # search for any program)
 CONFIG = {"hip.cell_scale": 2, "hip.tile": 1, "hip.block": 256, "hip.lds_slack": 1.25,
           "hip.nlist": 1, "hip.skin": 0.05}
+
+
+def parse_value(v: str):
+    """A launch-configuration value from the command line: int, float or text."""
+    for cast in (int, float):
+        try:
+            return cast(v)
+        except ValueError:
+            pass
+    return v
+
+
+def run_autotune(steps: int = 8, warmup: int = 3) -> dict:
+    """Time the autotuner's default space (threads per block, cell scale, tile
+    shape, LDS budget) on the metric workload's initial state, FP64, keeping
+    the list settings of CONFIG, and make the fastest the launch configuration."""
+    from openabl_amd import MEMBERS, Simulation, load_library, record_dtype
+    from openabl_amd.autotune import tune
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    sim = Simulation("boids2d", "f64", params=PARAMS, config=CONFIG)
+    sim.init_agents()
+    recs = [sim.download(record_dtype(MEMBERS["boids2d"], "f64"))]
+    sim.close()
+    t0 = time.perf_counter()
+    base = {k: v for k, v in CONFIG.items() if k in ("hip.nlist", "hip.skin")}
+    best, table = tune("boids2d", "f64", PARAMS, recs, steps=steps, warmup=warmup,
+                       base_config=base)
+    CONFIG.clear()
+    CONFIG.update(best)
+    ok = [row for row in table if row[1] is not None]
+    return {"best": best, "candidates": len(table), "ran": len(ok),
+            "best_ms_per_step": round(min(row[1] for row in ok) * 1e3, 4) if ok else None,
+            "steps_per_candidate": steps, "seconds": round(time.perf_counter() - t0, 1)}
 
 
 def progress(msg: str) -> None:
@@ -446,6 +483,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the runs, copy 64 MiB device to device 5 times (the HIP "
                          "runtime's copy kernel: reference for scaling HBM byte counters)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="override a key of the launch configuration (hip.*), e.g. hip.skin=0.04")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the launch configuration with openabl_amd.autotune on this "
+                         "workload before the timed runs")
     ap.add_argument("--mode", choices=["main", "ensemble", "serving", "precision"], default="main")
     ap.add_argument("--prec-steps", type=int, default=100)
     ap.add_argument("--ens-steps", type=int, default=500)
@@ -470,6 +512,12 @@ def main():
         print(json.dumps(run_precision(args)))
         return
 
+    for kv in args.set:
+        k, _, v = kv.partition("=")
+        CONFIG[k] = parse_value(v)
+    tuned = run_autotune() if args.autotune else None
+    if tuned:
+        progress(f"autotune: {tuned['best']} of {tuned['candidates']} in {tuned['seconds']} s")
     f64 = run_gpu("f64", args.steps, args.warmup, args.timing_steps)
     progress(f"f64: {f64['agent_steps_per_s']:.4g} agent-steps/s")
     f32 = None if args.no_fp32 else run_gpu("f32", args.steps, args.warmup, args.timing_steps)
@@ -501,6 +549,7 @@ def main():
     line = {
         "metric": "agent-steps/sec, 1M boids2d FP64 vs FP32",
         "launch_config": CONFIG,
+        "autotune": tuned,
         "value": round(f64["agent_steps_per_s"], 1),
         "unit": "agent-steps/s",
         "n_gpus": 1,
