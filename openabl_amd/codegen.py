@@ -113,6 +113,8 @@ def _programs():
               "game_of_life", "predator_prey", "boids", "boids2d_flockers", "ants", "sugarscape",
               "keratinocyte"):
         out[f"model_{m}"] = (m, {}, {})
+    # boids2d_cfg with its near loop on the runtime's tiled neighbour lists
+    out["model_boids2d_lists"] = ("boids2d_cfg", {}, {"hip.lists": "1"})
     return out
 
 

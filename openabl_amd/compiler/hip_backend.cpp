@@ -227,6 +227,14 @@ class Gen {
   }
   std::string load_member(const AgentDecl *ag, int m, const std::string &dst,
                           const std::string &cols, const std::string &slot);
+  // a near-loop candidate's member from its record accessor nb_ (column k at
+  // record position staged[k])
+  std::string load_member_nb(const AgentDecl *ag, int m, const std::string &dst,
+                             const std::vector<int> &staged);
+  // the columns of the near agent a near loop reads: its position first
+  std::vector<int> staged_cols(const AgentDecl *nag, const std::set<std::string> &need);
+  // whether step f can run on the runtime's tiled neighbour lists
+  bool list_capable(const FuncDecl &f);
   std::string store_member(const AgentDecl *ag, int m, const std::string &src,
                            const std::string &cols, const std::string &slot);
   bool calls_rng(const FuncDecl &f) { return f.uses_rng; }
@@ -566,6 +574,76 @@ std::string Gen::load_member(const AgentDecl *ag, int m, const std::string &dst,
   return s;
 }
 
+std::string Gen::load_member_nb(const AgentDecl *ag, int m, const std::string &dst,
+                                const std::vector<int> &staged) {
+  const auto &cs = agent_cols.at(ag);
+  const Type t = ag->members[m]->type.resolved;
+  std::string s;
+  const char *comp[] = {"x", "y", "z"};
+  for (size_t k = 0; k < cs.size(); k++) {
+    if (cs[k].member != m) continue;
+    const std::string target = dst + "." + ag->members[m]->name +
+                               (t.is_vec() ? std::string(".") + comp[cs[k].comp] : "");
+    int slot = -1;
+    for (size_t q = 0; q < staged.size(); q++)
+      if (staged[q] == (int) k) slot = (int) q;
+    const std::string args = "(" + std::to_string(k) + ", " + std::to_string(slot) + ")";
+    if (t.is_bool())
+      s += target + " = nb_.template get<uint8_t>" + args + " != 0; ";
+    else
+      s += target + " = nb_.template get<" + cs[k].ctype + ">" + args + "; ";
+  }
+  return s;
+}
+
+std::vector<int> Gen::staged_cols(const AgentDecl *nag, const std::set<std::string> &need) {
+  const auto &cs = agent_cols.at(nag);
+  const AgentMemberDecl *pos = nag->position();
+  std::vector<int> r;
+  for (size_t k = 0; k < cs.size(); k++)
+    if (nag->members[cs[k].member].get() == pos) r.push_back((int) k);
+  for (size_t k = 0; k < cs.size(); k++) {
+    const AgentMemberDecl *md = nag->members[cs[k].member].get();
+    if (md != pos && need.count(md->name)) r.push_back((int) k);
+  }
+  return r;
+}
+
+// A step runs on the tiled neighbour lists when the program is generated
+// with -C hip.lists=1, its one near loop walks the stepping agent's own type
+// around the stepping agent itself, it neither adds nor removes agents, and
+// the loop reads only float members of the candidates (the records staged in
+// LDS are of the program's float type).
+bool Gen::list_capable(const FuncDecl &f) {
+  auto on = ctx.config.find("hip.lists");
+  if (on == ctx.config.end() || (on->second != "1" && on->second != "true")) return false;
+  AgentDecl *ag = f.step_agent();
+  if (!f.near_agent || f.near_agent != ag || f.uses_removal || f.added_agent) return false;
+  int loops = 0;
+  const Stmt *nl_stmt = nullptr;
+  for (auto &st : f.body)
+    each_stmt(
+        *st,
+        [&](const Stmt &x) {
+          if (x.for_kind == ForKind::Near) {
+            loops++;
+            nl_stmt = &x;
+          }
+        },
+        [](const Expr &) {});
+  if (loops != 1) return false;
+  const Expr &centre = *nl_stmt->e0->a[0];
+  if (centre.kind != EK::Var || centre.var.id != f.params[0]->var.id) return false;
+  std::set<std::string> need = f.near_members;
+  need.insert(ag->position()->name);
+  for (auto &md : ag->members) {
+    if (!need.count(md->name)) continue;
+    const Type t = md->type.resolved;
+    if (!(t.is_float() || t.is_vec())) return false;
+  }
+  return true;
+}
+
 std::string Gen::store_member(const AgentDecl *ag, int m, const std::string &src,
                               const std::string &cols, const std::string &slot) {
   const auto &cs = agent_cols.at(ag);
@@ -618,16 +696,18 @@ void Gen::near_loop(const Stmt &s) {
   o << nl() << "const auto " << p << " = " << expr(centre_agent) << "." << pos->name << ";";
   if (has_return) o << nl() << "bool ret_ = false;";
   o << nl() << "c_.near(" << p << ".x, " << p << ".y, " << (dim3 ? p + ".z" : "(F) 0")
-    << ", [&](uint32_t j_) -> bool {";
+    << ", [&](const auto &nb_) -> bool {";
   ind++;
   const std::string v = ident(s.var.name);
   o << nl() << nag->name << "<F> " << v << ";";
-  // the near agent's members the loop reads, and its position
+  // the near agent's members the loop reads, and its position (from the
+  // columns, or from the staged record of a list walk)
   std::set<std::string> need = fn ? fn->near_members : std::set<std::string>{};
   need.insert(pos->name);
+  const std::vector<int> staged = staged_cols(nag, need);
   o << nl();
   for (size_t m = 0; m < nag->members.size(); m++)
-    if (need.count(nag->members[m]->name)) o << load_member(nag, (int) m, v, "c_.a.nb", "j_");
+    if (need.count(nag->members[m]->name)) o << load_member_nb(nag, (int) m, v, staged);
   o << nl() << "if (c_.outside(" << v << "." << pos->name << ", " << p << ")) return true;";
   loops.push_back('n');
   if (s.s0->kind == SK::Block) body(s.s0->body);
@@ -648,7 +728,7 @@ void Gen::simulate(const Stmt &s) {
   ind++;
   o << nl() << "abl_sim *sim_ = abl_gen_start(\"" << prog_name << "\", &agents, agents_info);";
   for (const auto &kv : ctx.config) {
-    if (kv.first.rfind("hip.", 0) == 0 && kv.first != "hip.program")
+    if (kv.first.rfind("hip.", 0) == 0 && kv.first != "hip.program" && kv.first != "hip.lists")
       o << nl() << "abl_gen_config(sim_, " << string_lit(kv.first) << ", " << string_lit(kv.second)
         << ");";
   }
@@ -995,8 +1075,41 @@ std::string Gen::device_program() {
       if (written) o << store_member(ag, (int) m, "out_", "a.out", "i");
     }
     o << "\n}\n";
-    o << "template <typename F>\nvoid launch_" << f->name << "_t(const StepLaunch &L) {\n"
-      << "  const GenArgs a = make_args(L);\n"
+    const bool lists = list_capable(*f);
+    if (lists) {
+      // the step on the runtime's tiled neighbour lists (abl_gen_device.hpp)
+      std::set<std::string> need = f->near_members;
+      need.insert(ag->position()->name);
+      const std::vector<int> staged = staged_cols(ag, need);
+      AgentMemberDecl *pm = ag->position();
+      const bool d3 = pm->type.resolved.id == TI::Vec3;
+      o << "template <typename F_>\nstruct Step_" << f->name << " {\n"
+        << "  using F = F_;\n  using Agent = " << ag->name << "<F>;\n"
+        << "  static constexpr int DIM = " << (d3 ? 3 : 2) << ";\n"
+        << "  static constexpr int NS = " << staged.size() << ";\n"
+        << "  static constexpr int kCols[NS] = {";
+      for (size_t q = 0; q < staged.size(); q++) o << (q ? ", " : "") << staged[q];
+      o << "};\n"
+        << "  static __device__ Agent load(const GenArgs &a, int i) { return load_" << ag->name
+        << "<F>(a.in, i); }\n"
+        << "  static __device__ void store(const GenArgs &a, int i, const Agent &out_) {\n    ";
+      for (size_t m = 0; m < ag->members.size(); m++) {
+        bool written = false;
+        for (size_t c = 0; c < cs.size(); c++)
+          if (cs[c].member == (int) m && ((w >> c) & 1u)) written = true;
+        if (written) o << store_member(ag, (int) m, "out_", "a.out", "i");
+      }
+      o << "\n  }\n"
+        << "  static __device__ void pos(const Agent &o, double *p) { p[0] = (double) o."
+        << pm->name << ".x; p[1] = (double) o." << pm->name << ".y;"
+        << (d3 ? " p[2] = (double) o." + pm->name + ".z;" : std::string()) << " }\n"
+        << "  template <class C>\n  static __device__ void run(C &c_, const Agent &in, Agent &out) {\n"
+        << "    Prog<F>::" << ident(f->name) << "(c_, in, out);\n  }\n};\n";
+    }
+    o << "template <typename F>\nvoid launch_" << f->name << "_t(const StepLaunch &L) {\n";
+    if (lists)
+      o << "  if (L.nl_mode) {\n    launch_gen_lists<Step_" << f->name << "<F>>(L);\n    return;\n  }\n";
+    o << "  const GenArgs a = make_args(L);\n"
       << "  const unsigned nb = grid_blocks(L.n, 256);\n"
       << "  if (L.order == 1) hipLaunchKernelGGL((" << k << "<F, true>), nb, 256, 0, L.stream, a);\n"
       << "  else hipLaunchKernelGGL((" << k << "<F, false>), nb, 256, 0, L.stream, a);\n}\n"
@@ -1055,8 +1168,9 @@ std::string Gen::device_program() {
     const int adds = f->added_agent ? 1 + f->added_agent->index : 0;
     o << (k ? ",\n     " : "") << "{\"" << f->name << "\", " << f->step_agent()->index << ", "
       << (f->near_agent ? f->near_agent->index : -1) << ", nullptr, 0.0, " << w << "u, launch_"
-      << f->name << ", " << (f->uses_removal ? "true" : "false") << ", " << adds
-      << ", false, " << (f->near_agent ? "radius_" + f->name : std::string("nullptr")) << "}";
+      << f->name << ", " << (f->uses_removal ? "true" : "false") << ", " << adds << ", "
+      << (list_capable(*f) ? "true" : "false") << ", "
+      << (f->near_agent ? "radius_" + f->name : std::string("nullptr")) << "}";
   }
   o << "},\n    env,\n    nullptr,\n    nullptr,\n    nullptr,\n    false,\n    "
     << (S.simulate && S.simulate->seq_decl ? "seq_step" : "nullptr") << ",\n  };\n  return &d;\n}\n\n"

@@ -277,6 +277,19 @@ struct policy_relative<M, decltype((void) M::kRelative)> {
   static constexpr bool value = M::kRelative;
 };
 
+// A policy whose agent's whole step runs as one function (the generated
+// programs): M::agent(a, gi, walk, np) runs agent gi's step, handing the
+// step's near loop `walk`, which calls visit(nb) for every candidate record
+// in order (visit returns false to stop), and leaves the new position in np.
+template <class M, class = void>
+struct policy_agent {
+  static constexpr bool value = false;
+};
+template <class M>
+struct policy_agent<M, decltype((void) M::kAgentStep)> {
+  static constexpr bool value = M::kAgentStep;
+};
+
 template <class M, int TX, int TY, int TZ, int RM>
 __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned tile,
                                            unsigned char *smem);
@@ -1105,27 +1118,52 @@ __device__ __forceinline__ bool step_list_tile(const typename M::Args &a, const 
       } else {
         gi = tile_own_src<M, OVF>(a, c, k, own, pd);
       }
-      typename M::Acc acc;
-      if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
-      else M::begin(a, acc, own);
-      if (len != kListOverflow) {
-        auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
-        list_walk<M, OVF>(a, c, nl, gi, len, row0, fn);
-      } else {
-        stray = true;
-        float u[3];
-        const int3 cell = own_cell<M::DIM>(a.g, pd, u);
-        auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
-        tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
-      }
-      // the build's position, read before the step's stores so its latency
-      // overlaps the epilogue
       double p0[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
       double np[3] = {0.0, 0.0, 0.0};
-      if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, np);
-      else M::finish(a, acc, own, gi, np);
+      if constexpr (policy_agent<M>::value) {
+        if (len == kListOverflow) stray = true;
+        // the agent's step with its near loop over the list (or, for an
+        // overflowed list, the stencil); a visit that returns false ends it
+        auto walk = [&](auto &&visit) {
+          bool go = true;
+          if (len != kListOverflow) {
+            auto fn = [&](const F *nb) {
+              if (go) go = visit(nb);
+            };
+            list_walk<M, OVF>(a, c, nl, gi, len, row0, fn);
+          } else {
+            float u[3];
+            const int3 cell = own_cell<M::DIM>(a.g, pd, u);
+            auto fn = [&](const F *nb, uint32_t) {
+              if (go) go = visit(nb);
+            };
+            tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+          }
+        };
+#pragma unroll
+        for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
+        M::agent(a, gi, walk, np);
+      } else {
+        typename M::Acc acc;
+        if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
+        else M::begin(a, acc, own);
+        if (len != kListOverflow) {
+          auto fn = [&](const F *nb) { acc.visit(a, own, nb); };
+          list_walk<M, OVF>(a, c, nl, gi, len, row0, fn);
+        } else {
+          stray = true;
+          float u[3];
+          const int3 cell = own_cell<M::DIM>(a.g, pd, u);
+          auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
+          tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+        }
+        // the build's position, read before the step's stores so its
+        // latency overlaps the epilogue
+#pragma unroll
+        for (int d = 0; d < DIM; d++) p0[d] = nl.pos0[(size_t) d * nl.n + gi];
+        if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, np);
+        else M::finish(a, acc, own, gi, np);
+      }
       double d2 = 0.0;
 #pragma unroll
       for (int d = 0; d < DIM; d++) {

@@ -187,9 +187,21 @@ def test_step_program_shape(tmp_path):
     assert "c_.near(" in dev and "c_.outside(nx.pos, " in dev
     assert "static __host__ __device__ double radius_move() { return (double) ((2.0 * r)); }" in dev
     assert '{"move", 0, 0, nullptr, 0.0, 3u, launch_move, false, 0, false, radius_move}' in dev
+    assert "[&](const auto &nb_) -> bool {" in dev and "struct Step_move" not in dev
     host = (out / "main.cpp").read_text()
     assert 'abl_gen_config(sim_, "hip.order", "index");' in host
     assert 'abl_save_json(&agents, agents_info, "out.json", (int) sizeof(abl_float));' in host
+    # with -C hip.lists=1 the near loop (over the stepping agent's own type,
+    # around it, float members only) runs on the runtime's tiled lists; the
+    # key is the generator's, not passed on to the runtime
+    out2 = tmp_path / "s2"
+    r = ablc("-i", model(tmp_path, CIRCLE_MIN), "-b", "hip", "-o", str(out2),
+             "-C", "hip.program=circle_min", "-C", "hip.lists=1")
+    assert r.returncode == 0, r.stderr
+    dev = (out2 / "steps.hip").read_text()
+    assert '{"move", 0, 0, nullptr, 0.0, 3u, launch_move, false, 0, true, radius_move}' in dev
+    assert "struct Step_move {" in dev and "launch_gen_lists<Step_move<F>>(L);" in dev
+    assert "hip.lists" not in (out2 / "main.cpp").read_text()
 
 
 def test_copy_of_input_member_is_not_written(tmp_path):
@@ -213,6 +225,13 @@ void main() {
     assert ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out)).returncode == 0
     dev = (out / "steps.hip").read_text()
     assert '{"s", 0, 0, nullptr, 0.0, 4u, launch_s,' in dev  # only the alive column
+    # the near loop reads a bool member: even with -C hip.lists=1 the step
+    # walks the grid, not the lists
+    out2 = tmp_path / "c2"
+    assert ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out2),
+                "-C", "hip.lists=1").returncode == 0
+    dev = (out2 / "steps.hip").read_text()
+    assert "launch_s, false, 0, false, radius_s}" in dev and "struct Step_s" not in dev
 
 
 def test_generated_main_links_step_library():
