@@ -233,7 +233,9 @@ class Gen {
                              const std::vector<int> &staged);
   // the columns of the near agent a near loop reads: its position first
   std::vector<int> staged_cols(const AgentDecl *nag, const std::set<std::string> &need);
-  // whether step f can run on the runtime's tiled neighbour lists
+  // whether step f can run on the runtime's LDS tiles (its near loop as a
+  // walk of each agent's stencil), and on its neighbour lists (opt-in)
+  bool tile_capable(const FuncDecl &f);
   bool list_capable(const FuncDecl &f);
   std::string store_member(const AgentDecl *ag, int m, const std::string &src,
                            const std::string &cols, const std::string &slot);
@@ -609,14 +611,18 @@ std::vector<int> Gen::staged_cols(const AgentDecl *nag, const std::set<std::stri
   return r;
 }
 
-// A step runs on the tiled neighbour lists when the program is generated
-// with -C hip.lists=1, its one near loop walks the stepping agent's own type
-// around the stepping agent itself, it neither adds nor removes agents, and
-// the loop reads only float members of the candidates (the records staged in
-// LDS are of the program's float type).
+// A step runs on the runtime's LDS tiles when its one near loop walks the
+// stepping agent's own type around the stepping agent itself, it neither
+// adds nor removes agents, and the loop reads only float members of the
+// candidates (the records staged in LDS are of the program's float type); on
+// the neighbour lists too when the program is generated with -C hip.lists=1.
 bool Gen::list_capable(const FuncDecl &f) {
   auto on = ctx.config.find("hip.lists");
   if (on == ctx.config.end() || (on->second != "1" && on->second != "true")) return false;
+  return tile_capable(f);
+}
+
+bool Gen::tile_capable(const FuncDecl &f) {
   AgentDecl *ag = f.step_agent();
   if (!f.near_agent || f.near_agent != ag || f.uses_removal || f.added_agent) return false;
   int loops = 0;
@@ -1075,9 +1081,10 @@ std::string Gen::device_program() {
       if (written) o << store_member(ag, (int) m, "out_", "a.out", "i");
     }
     o << "\n}\n";
+    const bool tiles = tile_capable(*f);
     const bool lists = list_capable(*f);
-    if (lists) {
-      // the step on the runtime's tiled neighbour lists (abl_gen_device.hpp)
+    if (tiles) {
+      // the step on the runtime's LDS tiles and lists (abl_gen_device.hpp)
       std::set<std::string> need = f->near_members;
       need.insert(ag->position()->name);
       const std::vector<int> staged = staged_cols(ag, need);
@@ -1109,6 +1116,9 @@ std::string Gen::device_program() {
     o << "template <typename F>\nvoid launch_" << f->name << "_t(const StepLaunch &L) {\n";
     if (lists)
       o << "  if (L.nl_mode) {\n    launch_gen_lists<Step_" << f->name << "<F>>(L);\n    return;\n  }\n";
+    if (tiles)
+      o << "  if (L.tile != 0 && L.order == 0 && L.grid.reach <= 4) {\n    launch_gen_tiles<Step_"
+        << f->name << "<F>>(L);\n    return;\n  }\n";
     o << "  const GenArgs a = make_args(L);\n"
       << "  const unsigned nb = grid_blocks(L.n, 256);\n"
       << "  if (L.order == 1) hipLaunchKernelGGL((" << k << "<F, true>), nb, 256, 0, L.stream, a);\n"

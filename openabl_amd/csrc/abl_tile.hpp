@@ -826,13 +826,26 @@ __device__ __forceinline__ void tiled_body(const typename M::Args &a, unsigned t
       float u[3];
       const uint32_t gi = tile_own_src<M, OVF>(a, c, k, own, pd);
       const int3 cell = own_cell<M::DIM>(a.g, pd, u);
-      typename M::Acc acc;
-      if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
-      else M::begin(a, acc, own);
-      auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
-      tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
-      if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, nullptr);
-      else M::finish(a, acc, own, gi, nullptr);
+      if constexpr (policy_agent<M>::value) {
+        // the agent's step with its near loop over the stencil's records
+        auto walk = [&](auto &&visit) {
+          bool go = true;
+          auto fn = [&](const F *nb, uint32_t) {
+            if (go) go = visit(nb);
+          };
+          tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+        };
+        double np[3];
+        M::agent(a, gi, walk, np);
+      } else {
+        typename M::Acc acc;
+        if constexpr (policy_relative<M>::value) M::begin(a, acc, own, pd, c.org);
+        else M::begin(a, acc, own);
+        auto fn = [&](const F *nb, uint32_t) { acc.visit(a, own, nb); };
+        tile_walk_src<M, OVF>(a, c, cell, u, a.rc, fn);
+        if constexpr (policy_relative<M>::value) M::finish(a, acc, own, gi, pd, nullptr);
+        else M::finish(a, acc, own, gi, nullptr);
+      }
     }
   };
   if (c.ovf) agents(std::true_type{});

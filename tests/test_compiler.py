@@ -186,8 +186,11 @@ def test_step_program_shape(tmp_path):
     assert 'registers program "circle_min"' in dev and '"circle_min",' in dev
     assert "c_.near(" in dev and "c_.outside(nx.pos, " in dev
     assert "static __host__ __device__ double radius_move() { return (double) ((2.0 * r)); }" in dev
+    # the near loop walks the stepping agent's own type around it and reads
+    # float members only: the step runs on the runtime's LDS tiles
     assert '{"move", 0, 0, nullptr, 0.0, 3u, launch_move, false, 0, false, radius_move}' in dev
-    assert "[&](const auto &nb_) -> bool {" in dev and "struct Step_move" not in dev
+    assert "[&](const auto &nb_) -> bool {" in dev and "struct Step_move {" in dev
+    assert "launch_gen_tiles<Step_move<F>>(L);" in dev and "launch_gen_lists" not in dev
     host = (out / "main.cpp").read_text()
     assert 'abl_gen_config(sim_, "hip.order", "index");' in host
     assert 'abl_save_json(&agents, agents_info, "out.json", (int) sizeof(abl_float));' in host
@@ -226,7 +229,7 @@ void main() {
     dev = (out / "steps.hip").read_text()
     assert '{"s", 0, 0, nullptr, 0.0, 4u, launch_s,' in dev  # only the alive column
     # the near loop reads a bool member: even with -C hip.lists=1 the step
-    # walks the grid, not the lists
+    # walks the grid, neither tiles nor lists
     out2 = tmp_path / "c2"
     assert ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out2),
                 "-C", "hip.lists=1").returncode == 0
