@@ -233,6 +233,18 @@ class Gen {
                              const std::vector<int> &staged);
   // the columns of the near agent a near loop reads: its position first
   std::vector<int> staged_cols(const AgentDecl *nag, const std::set<std::string> &need);
+  // distances and lengths the device code only compares with constants:
+  // kept as their float square sums, compared against exact bounds
+  // (abl_root_bounds.hpp) instead of taking the root
+  std::set<int> sq_vars;  // locals of the function being emitted
+  static bool root_call(const Expr &e) {
+    return e.kind == EK::Call && e.call == CallKind::Builtin &&
+           (e.sig == "dist_float2" || e.sig == "dist_float3" || e.sig == "length_float2" ||
+            e.sig == "length_float3");
+  }
+  bool const_expr(const Expr &e);
+  void find_sq_vars(const FuncDecl &f);
+  std::string root_compare(const Expr &e);
   // whether step f can run on the runtime's LDS tiles (its near loop as a
   // walk of each agent's stencil), and on its neighbour lists (opt-in)
   bool tile_capable(const FuncDecl &f);
@@ -291,6 +303,10 @@ std::string Gen::expr(const Expr &e) {
     case EK::Binary: {
       const Expr &l = *e.a[0], &r = *e.a[1];
       if (l.type.is_vec() || r.type.is_vec()) return vec_op(e.bop, l, r);
+      if (mode == Mode::Device) {
+        const std::string rc = root_compare(e);
+        if (!rc.empty()) return rc;
+      }
       if (e.bop == BinOp::Mod && !(l.type.is_int() && r.type.is_int()))
         return "fmod((double) (" + expr(l) + "), (double) (" + expr(r) + "))";
       if (e.bop == BinOp::Range) throw BackendError("a range outside a for loop");
@@ -494,6 +510,15 @@ void Gen::stmt(const Stmt &s) {
         o << ";" << nl() << t.agent->name << " *" << ident(s.var.name) << " = &" << st << ";";
         return;
       }
+      if (mode == Mode::Device && sq_vars.count(s.var.id)) {
+        // only compared with constants: the square sum, no root
+        const Expr &c = *s.e0;
+        std::string a;
+        for (size_t k = 0; k < c.a.size(); k++) a += (k ? ", " : "") + expr(*c.a[k]);
+        const std::string n = c.sig.substr(0, c.sig.find('_')) + (c.sig.back() == '2' ? "2" : "3");
+        o << nl() << "const SqLen<F> " << ident(s.var.name) << " = sq_" << n << "<F>(" << a << ");";
+        return;
+      }
       o << nl() << type_name(t) << " " << ident(s.var.name);
       if (s.e0) o << " = " << expr(*s.e0);
       o << ";";
@@ -648,6 +673,92 @@ bool Gen::tile_capable(const FuncDecl &f) {
     if (!(t.is_float() || t.is_vec())) return false;
   }
   return true;
+}
+
+bool Gen::const_expr(const Expr &e) {
+  bool constant = true;
+  each_expr(e, [&](const Expr &x) {
+    if (x.kind == EK::Var && !global_of(x)) constant = false;
+    if (x.kind == EK::Member || (x.kind == EK::Call && x.call != CallKind::Ctor &&
+                                 x.call != CallKind::Builtin))
+      constant = false;
+    if (x.kind == EK::Call && x.call == CallKind::Builtin &&
+        (x.sig.rfind("random", 0) == 0 || x.sig == "count" || x.sig == "sum"))
+      constant = false;
+  });
+  return constant && !e.type.is_vec();
+}
+
+static bool relational(BinOp op) {
+  return op == BinOp::Lt || op == BinOp::Le || op == BinOp::Gt || op == BinOp::Ge;
+}
+
+// A float local initialised with dist()/length(), never assigned again, and
+// used only as an operand of <, <=, > or >= against a constant expression.
+void Gen::find_sq_vars(const FuncDecl &f) {
+  sq_vars.clear();
+  std::map<int, int> uses, good;
+  std::set<int> cand, assigned;
+  std::function<void(const Expr &)> walk = [&](const Expr &x) {
+    if (x.kind == EK::Var) uses[x.var.id]++;
+    if (x.kind == EK::Binary && relational(x.bop)) {
+      const Expr &l = *x.a[0], &r = *x.a[1];
+      if (l.kind == EK::Var && const_expr(r)) good[l.var.id]++;
+      else if (r.kind == EK::Var && const_expr(l)) good[r.var.id]++;
+    }
+    for (auto &y : x.a) walk(*y);
+    for (auto &m : x.inits) walk(*m.expr);
+  };
+  for (auto &st : f.body)
+    each_stmt(
+        *st,
+        [&](const Stmt &x) {
+          if (x.kind == SK::VarDecl && x.e0 && x.type.resolved.is_float() && root_call(*x.e0))
+            cand.insert(x.var.id);
+          if ((x.kind == SK::Assign || x.kind == SK::AssignOp) && x.e0 && x.e0->kind == EK::Var)
+            assigned.insert(x.e0->var.id);
+          // the statement's own expressions (each_stmt's expression callback
+          // would visit every subexpression, without the parent relation)
+          if (x.e0 && !(x.kind == SK::Assign || x.kind == SK::AssignOp)) walk(*x.e0);
+          if (x.kind == SK::Assign || x.kind == SK::AssignOp) {
+            if (x.e0 && x.e0->kind != EK::Var) walk(*x.e0);
+          }
+          if (x.e1) walk(*x.e1);
+        },
+        [](const Expr &) {});
+  for (int v : cand)
+    if (!assigned.count(v) && uses[v] > 0 && uses[v] == good[v]) sq_vars.insert(v);
+}
+
+// `l op r` with a square-sum local or a dist()/length() call on one side and
+// a constant on the other: the exact bound comparison (abl_root_bounds.hpp).
+// The constant is converted as C does: an int to abl_float, a float operand
+// compared in its own (double or promoted) value.
+std::string Gen::root_compare(const Expr &e) {
+  if (!relational(e.bop)) return "";
+  const Expr &l = *e.a[0], &r = *e.a[1];
+  auto sq_side = [&](const Expr &x) {
+    return (x.kind == EK::Var && sq_vars.count(x.var.id)) || root_call(x);
+  };
+  const bool left = sq_side(l) && const_expr(r);
+  const bool right = !left && sq_side(r) && const_expr(l);
+  if (!left && !right) return "";
+  const Expr &q = left ? l : r;
+  const Expr &c = left ? r : l;
+  BinOp op = e.bop;
+  if (right) op = op == BinOp::Lt ? BinOp::Gt : op == BinOp::Gt ? BinOp::Lt
+                : op == BinOp::Le ? BinOp::Ge : BinOp::Le;
+  std::string qs;
+  if (q.kind == EK::Var) {
+    qs = ident(q.var.name);
+  } else {
+    std::string a;
+    for (size_t k = 0; k < q.a.size(); k++) a += (k ? ", " : "") + expr(*q.a[k]);
+    qs = "sq_" + q.sig.substr(0, q.sig.find('_')) + (q.sig.back() == '2' ? "2" : "3") + "<F>(" + a + ")";
+  }
+  const std::string cv = c.type.is_int() ? "(double) (F) (" + expr(c) + ")" : "(double) (" + expr(c) + ")";
+  const char *fnm = op == BinOp::Lt ? "len_lt" : op == BinOp::Le ? "len_le" : op == BinOp::Gt ? "len_gt" : "len_ge";
+  return std::string(fnm) + "<F>(" + qs + ", " + cv + ")";
 }
 
 std::string Gen::store_member(const AgentDecl *ag, int m, const std::string &src,
@@ -995,6 +1106,7 @@ std::string Gen::device_program() {
          << "(" << func_params(*f) << ") {";
     }
     ind = 2;
+    find_sq_vars(*f);
     try {
       body(f->body);
     } catch (const BackendError &) {

@@ -237,6 +237,46 @@ void main() {
     assert "launch_s, false, 0, false, radius_s}" in dev and "struct Step_s" not in dev
 
 
+def test_distance_compared_with_constant_takes_no_root(tmp_path):
+    """A float local holding dist()/length() that is only compared with
+    constants is kept as the float square sum and compared against exact
+    bounds; one used in arithmetic, or compared with a local, keeps the
+    root."""
+    src = """
+agent P { position float2 pos; float2 v; }
+environment { max: float2(10) }
+param float R = 2;
+step s(P in -> out) {
+  int n = 0;
+  float2 acc = float2(0);
+  for (P q : near(in, 3)) {
+    float d = dist(in.pos, q.pos);
+    if (d < R && 1 <= d) n += 1;
+    float e = dist(in.pos, q.pos);
+    if (e > 0.5) acc += (q.pos - in.pos) / e;
+  }
+  float sp = length(in.v);
+  float lim = 1.5;
+  if (sp > lim) out.v = in.v / sp;
+  if (length(in.v) >= 2 * R) out.v = float2(0);
+  out.pos = in.pos + acc * n;
+}
+void main() {
+  for (int i : 0..10) { add(P { pos: float2(i, 0), v: float2(1, 0) }); }
+  simulate(3) { s }
+}
+"""
+    out = tmp_path / "d"
+    r = ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out))
+    assert r.returncode == 0, r.stderr
+    dev = (out / "steps.hip").read_text()
+    assert "const SqLen<F> d = sq_dist2<F>(in.pos, q.pos);" in dev
+    assert "len_lt<F>(d, (double) (R))" in dev and "len_ge<F>(d, (double) (F) (1))" in dev
+    assert "F e = dist2<F>(in.pos, q.pos);" in dev
+    assert "F sp = length2<F>(in.v);" in dev
+    assert "len_ge<F>(sq_length2<F>(in.v), (double) ((2.0 * R)))" in dev
+
+
 def test_generated_main_links_step_library():
     """The generated host program names no symbol of libsteps.so (the step
     library registers its program with the runtime when it is loaded), so
