@@ -29,6 +29,9 @@ The JSON line carries
                    first M agents' update_boid against the full 1M population;
   serving          p50/p99 run latency and aggregate agent-steps/s of a
                    Poisson stream of mixed runs (1k-1M agents, 100-1000 steps);
+  generated        the same workload as openabl-hip -b hip emits it
+                   (boids2d_cfg.abl at 1M agents, generated step on the LDS tiles),
+                   beside the hand-written kernels at the same launch configuration;
   ensemble         256 uneven boids2d members batched, against members alone;
   precision_circle3d  1M clustered circle3d: FP64, FP32-compensated and FP32
                    agent-steps/s and the per-step error against FP64;
@@ -289,6 +292,48 @@ def ensemble_members(k=256, seed=2026):
     return out
 
 
+def run_generated(steps: int, warmup: int):
+    """The metric workload as `openabl-hip -b hip` emits it: examples/boids2d_cfg.abl
+    generated with -P num_agents=1048576 (prebuilt by __graft_entry__.build()
+    as openabl_amd/gen/bench_boids2d_cfg), its update_boid on the runtime's
+    LDS tiles (the generated default), from the same initial state, FP64;
+    and its results after the timed steps against the hand-written kernels'
+    walking the same tiled stencil (bit for bit)."""
+    from openabl_amd import MEMBERS, Simulation, codegen, load_library, record_dtype
+    d = codegen.program_dir("bench_boids2d_cfg")
+    if not os.path.exists(os.path.join(d, "libsteps.so")):
+        return {"skipped": "bench_boids2d_cfg not built"}
+    codegen.load_program(d)
+    dt = record_dtype(MEMBERS["boids2d"], "f64")
+    load_library().abl_rng_seed(0xDEADBEEF, 0xBEEFDEAD)
+    s0 = Simulation("boids2d", "f64", params=PARAMS, config=CONFIG)
+    s0.init_agents()
+    rec = s0.download(dt)
+    s0.close()
+    cfg = {"hip.cell_scale": CONFIG["hip.cell_scale"], "hip.tile": CONFIG["hip.tile"],
+           "hip.block": CONFIG["hip.block"], "hip.lds_slack": CONFIG["hip.lds_slack"],
+           "hip.nlist": 0}
+    out = {}
+    for prog, params in (("bench_boids2d_cfg", None), ("boids2d", PARAMS)):
+        sim = Simulation(prog, "f64", params=params, config=cfg)
+        sim.upload(rec)
+        sim.simulate(warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.simulate(steps)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        out[prog] = (N_AGENTS * steps / wall, 1e3 * wall / steps, sim.download(dt).tobytes())
+        sim.close()
+    g, h = out["bench_boids2d_cfg"], out["boids2d"]
+    return {"metric": "agent-steps/s, 1M boids2d_cfg FP64 as openabl-hip -b hip generates it",
+            "value": round(g[0], 1), "ms_per_step": round(g[1], 5),
+            "handwritten_same_config": round(h[0], 1),
+            "results_equal_handwritten": g[2] == h[2],
+            "config": dict(cfg, program="examples/boids2d_cfg.abl -P num_agents=1048576",
+                           near_loop="LDS tiles, stencil walk (lists are opt-in: -C hip.lists=1)")}
+
+
 def run_ensemble(args):
     """256 uneven boids2d members, one launch per kernel, graph-replayed
     timesteps through neighbour lists (each member re-sorted and rebuilt on
@@ -478,6 +523,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-generated", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--error-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
@@ -586,6 +632,9 @@ def main():
     if f32c and args.error_steps > 0:
         line["fp32_compensated"]["error_vs_f64"] = boids_precision_error(args.error_steps)
         progress("f32c error leg done")
+    if not args.no_generated:
+        line["generated"] = run_generated(min(args.steps, 100), args.warmup)
+        progress("generated leg done")
     if not args.no_ensemble:
         line["ensemble"] = run_ensemble(args)
         progress("ensemble leg done")

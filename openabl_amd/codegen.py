@@ -115,6 +115,8 @@ def _programs():
         out[f"model_{m}"] = (m, {}, {})
     # boids2d_cfg with its near loop on the runtime's tiled neighbour lists
     out["model_boids2d_lists"] = ("boids2d_cfg", {}, {"hip.lists": "1"})
+    # the metric workload as the generator emits it (bench.py's generated leg)
+    out["bench_boids2d_cfg"] = ("boids2d_cfg", {"num_agents": 1048576}, {})
     return out
 
 
