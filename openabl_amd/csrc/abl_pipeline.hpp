@@ -137,8 +137,18 @@ __device__ __forceinline__ void scan_apply_tile(int tile, int ntiles,
                                                 const uint32_t *__restrict__ bsum,
                                                 uint32_t *__restrict__ start, int ncells,
                                                 uint32_t *lds) {
+  // the totals of the tiles before this one, read four at a time (bsum is
+  // 16-byte aligned) with several reads in flight per thread: on a grid of
+  // millions of cells a block adds up thousands of them
   uint32_t part = 0;
-  for (int t = threadIdx.x; t < tile; t += blockDim.x) part += bsum[t];
+  const int quads = tile >> 2;
+  const uint4 *b4 = reinterpret_cast<const uint4 *>(bsum);
+#pragma unroll 4
+  for (int t = threadIdx.x; t < quads; t += blockDim.x) {
+    const uint4 q = b4[t];
+    part += (q.x + q.y) + (q.z + q.w);
+  }
+  if ((int) threadIdx.x < (tile & 3)) part += bsum[(quads << 2) + threadIdx.x];
   uint32_t carry;
   (void) block_excl_scan(part, lds, &carry);
   const int base = tile * kScanTile + threadIdx.x * kScanItems;
@@ -201,6 +211,9 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
                      (uint32_t) c.x;
   key[i] = k;
   const uint32_t o = skey[i];
+  // the wave's movers take consecutive mover numbers with one atomic (the
+  // numbering is free: k_place ranks a cell's agents by id)
+  const uint64_t movers = __builtin_amdgcn_ballot_w64(k != o);
   if (k != o) {
     atomicAdd(&count[k], 1u);
     atomicSub(&count[o], 1u);
@@ -208,7 +221,12 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
     atomicSub(&bsum[o / kScanTile], 1u);
     ml.dirty[k] = gen;
     ml.dirty[o] = gen;
-    const uint32_t m = atomicAdd(&ml.nmov[gen & 1u], 1u);
+    const int leader = __builtin_ctzll(movers);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t) (movers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) movers, 0u));
+    uint32_t first = 0;
+    if (below == 0) first = atomicAdd(&ml.nmov[gen & 1u], (uint32_t) __builtin_popcountll(movers));
+    const uint32_t m = (uint32_t) __builtin_amdgcn_readlane((int) first, leader) + below;
     if (m < (uint32_t) n) {  // always true when the counter started at zero
       ml.slot[m] = (uint32_t) i;
       ml.next[m] = atomicExch(&ml.head[k], ((unsigned long long) gen << 32) | m);

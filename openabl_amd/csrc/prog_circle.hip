@@ -33,6 +33,10 @@ namespace {
 enum { P_NUM_TIMESTEPS, P_NUM_AGENTS, P_RHO, P_K_REP, P_K_ATT, P_R, P_W, P_NUM_CLUSTERS,
        P_CLUSTER_SIGMA, P_BACKGROUND_FRACTION };
 
+// Candidates of a stencil row read at once by the thread-per-agent kernels:
+// the walk waits on each read in turn, so a lane's reads are issued in batches.
+constexpr uint32_t kRowBatch = 4;
+
 template <typename F, int DIM>
 struct CircleArgs {
   const F *p[3];   // in.pos columns (cell-sorted)
@@ -126,14 +130,22 @@ __global__ __launch_bounds__(1024) void circle_step(CircleArgs<F, DIM> a) {
     ColBuf cb[3];
 #pragma unroll
     for (int d = 0; d < DIM; d++) cb[d] = col_buf(a.p[d], (uint32_t) a.n * (uint32_t) sizeof(F));
+    // kRowBatch candidates of a row are read together (their loads in flight
+    // at once) and then visited in order; reads past the row's end are not
+    // visited, and past the column's end the buffer returns zero
+    constexpr uint32_t S = (uint32_t) sizeof(F);
     for_each_candidate_row(a.g, (double) x[0], (double) x[1], DIM == 3 ? (double) x[2] : 0.0, a.rc,
                            [&](uint32_t j0, uint32_t j1) {
-                             for (uint32_t off = j0 * (uint32_t) sizeof(F);
-                                  off < j1 * (uint32_t) sizeof(F); off += (uint32_t) sizeof(F)) {
-                               F q[3] = {0, 0, 0};
+                             const uint32_t end = j1 * S;
+                             for (uint32_t off = j0 * S; off < end; off += kRowBatch * S) {
+                               F q[kRowBatch][3] = {};
 #pragma unroll
-                               for (int d = 0; d < DIM; d++) q[d] = col_load<F>(cb[d], off);
-                               circle_visit<F, DIM>(a, x, q, np);
+                               for (int u = 0; u < kRowBatch; u++)
+#pragma unroll
+                                 for (int d = 0; d < DIM; d++) q[u][d] = col_load<F>(cb[d], off + u * S);
+#pragma unroll
+                               for (int u = 0; u < kRowBatch; u++)
+                                 if (off + u * S < end) circle_visit<F, DIM>(a, x, q[u], np);
                              }
                            });
   }
@@ -240,12 +252,23 @@ __global__ __launch_bounds__(1024) void circle_step_c(CircleArgsC<DIM> a) {
     ColBuf cb[3];
 #pragma unroll
     for (int d = 0; d < DIM; d++) cb[d] = col_buf(a.p[d], (uint32_t) a.n * 8u);
+    // batches of kRowBatch reads, as circle_step
     for_each_candidate_row(a.g, x[0], x[1], x[2], a.rc, [&](uint32_t j0, uint32_t j1) {
-      for (uint32_t off = j0 * 8u; off < j1 * 8u; off += 8u) {
-        float o[3] = {0.f, 0.f, 0.f};
+      const uint32_t end = j1 * 8u;
+      for (uint32_t off = j0 * 8u; off < end; off += kRowBatch * 8u) {
+        double q[kRowBatch][3] = {};
 #pragma unroll
-        for (int d = 0; d < DIM; d++) o[d] = (float) (col_load<double>(cb[d], off) - x[d]);
-        s.visit(a, o);
+        for (int u = 0; u < kRowBatch; u++)
+#pragma unroll
+          for (int d = 0; d < DIM; d++) q[u][d] = col_load<double>(cb[d], off + u * 8u);
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+          if (off + u * 8u >= end) break;
+          float o[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int d = 0; d < DIM; d++) o[d] = (float) (q[u][d] - x[d]);
+          s.visit(a, o);
+        }
       }
     });
   }
