@@ -198,6 +198,23 @@ struct MoverLists {
   uint32_t *nmov;                  // two counters, used by alternate generations
 };
 
+// table[idx] += v for every active lane with `on`, one atomic per distinct
+// index among them (v is the same in every lane).  A wave holds consecutive
+// agents of the cell order, so its movers leave from one or two cells and
+// cross into a few scan tiles: per-lane atomics there all hit the same words.
+__device__ __forceinline__ void wave_add(uint32_t *table, uint32_t idx, bool on, uint32_t v) {
+  uint64_t left = __builtin_amdgcn_ballot_w64(on);
+  const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  while (left != 0) {
+    const int leader = __builtin_ctzll(left);
+    const uint32_t li = (uint32_t) __builtin_amdgcn_readlane((int) idx, leader);
+    const bool mine = on && idx == li;
+    const uint64_t same = __builtin_amdgcn_ballot_w64(mine);
+    if (lane == (uint32_t) leader) atomicAdd(&table[li], v * (uint32_t) __builtin_popcountll(same));
+    left &= ~same;
+  }
+}
+
 template <typename P>
 __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
                                                const P *__restrict__ py, const P *__restrict__ pz,
@@ -216,9 +233,12 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
   const uint64_t movers = __builtin_amdgcn_ballot_w64(k != o);
   if (k != o) {
     atomicAdd(&count[k], 1u);
-    atomicSub(&count[o], 1u);
-    atomicAdd(&bsum[k / kScanTile], 1u);
-    atomicSub(&bsum[o / kScanTile], 1u);
+    // leaving: few distinct cells per wave; tile totals change only for a
+    // mover that crosses into another scan tile
+    wave_add(count, o, true, ~0u);
+    const bool cross = k / kScanTile != o / kScanTile;
+    wave_add(bsum, k / kScanTile, cross, 1u);
+    wave_add(bsum, o / kScanTile, cross, ~0u);
     ml.dirty[k] = gen;
     ml.dirty[o] = gen;
     const int leader = __builtin_ctzll(movers);
