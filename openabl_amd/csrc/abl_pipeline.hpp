@@ -228,9 +228,6 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
                      (uint32_t) c.x;
   key[i] = k;
   const uint32_t o = skey[i];
-  // the wave's movers take consecutive mover numbers with one atomic (the
-  // numbering is free: k_place ranks a cell's agents by id)
-  const uint64_t movers = __builtin_amdgcn_ballot_w64(k != o);
   if (k != o) {
     atomicAdd(&count[k], 1u);
     // leaving: few distinct cells per wave; tile totals change only for a
@@ -241,16 +238,11 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
     wave_add(bsum, o / kScanTile, cross, ~0u);
     ml.dirty[k] = gen;
     ml.dirty[o] = gen;
-    const int leader = __builtin_ctzll(movers);
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t) (movers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) movers, 0u));
-    uint32_t first = 0;
-    if (below == 0) first = atomicAdd(&ml.nmov[gen & 1u], (uint32_t) __builtin_popcountll(movers));
-    const uint32_t m = (uint32_t) __builtin_amdgcn_readlane((int) first, leader) + below;
-    if (m < (uint32_t) n) {  // always true when the counter started at zero
-      ml.slot[m] = (uint32_t) i;
-      ml.next[m] = atomicExch(&ml.head[k], ((unsigned long long) gen << 32) | m);
-    }
+    // a mover's number is its slot (unique, below n: no shared counter; the
+    // numbering is free, k_place ranks a cell's agents by id)
+    const uint32_t m = (uint32_t) i;
+    ml.slot[m] = (uint32_t) i;
+    ml.next[m] = atomicExch(&ml.head[k], ((unsigned long long) gen << 32) | m);
   }
 }
 
