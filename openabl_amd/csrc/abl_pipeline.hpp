@@ -20,6 +20,41 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 static_assert(kScanItems % 4 == 0, "a thread's run of cells is read as 16-byte words");
 
+// A thread's kScanItems consecutive cells from `base` (zero past ncells):
+// 16-byte accesses when the run is whole and aligned, else one word each.
+__device__ __forceinline__ bool whole_run(const uint32_t *p, int base, int ncells) {
+  return base + kScanItems <= ncells && (reinterpret_cast<uintptr_t>(p + base) & 15u) == 0;
+}
+__device__ __forceinline__ void load_run(const uint32_t *__restrict__ count, int base, int ncells,
+                                         uint32_t *v) {
+  if (whole_run(count, base, ncells)) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(count + base);
+#pragma unroll
+    for (int j = 0; j < kScanItems / 4; j++) {
+      const uint4 x = q[j];
+      v[4 * j] = x.x;
+      v[4 * j + 1] = x.y;
+      v[4 * j + 2] = x.z;
+      v[4 * j + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) v[k] = base + k < ncells ? count[base + k] : 0u;
+  }
+}
+__device__ __forceinline__ void store_run(uint32_t *__restrict__ start, int base, int ncells,
+                                          const uint32_t *w) {
+  if (whole_run(start, base, ncells)) {
+    uint4 *q = reinterpret_cast<uint4 *>(start + base);
+#pragma unroll
+    for (int j = 0; j < kScanItems / 4; j++) q[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+      if (base + k < ncells) start[base + k] = w[k];
+  }
+}
+
 // Per tile: exclusive scan of count into start, tile total into bsum.
 // INV scans 1 - count (the survivors of a removeCurrent() flag array).
 // With a gate the sort kernels here run only while *gate is set.
@@ -32,20 +67,27 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const T *__res
   __shared__ uint32_t lds[kWave + 1];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
+  if constexpr (sizeof(T) == 4 && !INV) {
+    load_run(reinterpret_cast<const uint32_t *>(count), base, ncells, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      uint32_t c = base + k < ncells ? (uint32_t) count[base + k] : (INV ? 1u : 0u);
+      v[k] = INV ? 1u - c : c;
+    }
+  }
   uint32_t sum = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    uint32_t c = base + k < ncells ? (uint32_t) count[base + k] : (INV ? 1u : 0u);
-    v[k] = INV ? 1u - c : c;
-    sum += v[k];
-  }
+  for (int k = 0; k < kScanItems; k++) sum += v[k];
   uint32_t total;
   uint32_t pre = block_excl_scan(sum, lds, &total);
+  uint32_t w[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    if (base + k < ncells) start[base + k] = pre;
+    w[k] = pre;
     pre += v[k];
   }
+  store_run(start, base, ncells, w);
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
@@ -117,41 +159,6 @@ static __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const
 // k_tile_totals: the total of every tile of kScanTile cells.  k_scan_apply:
 // each tile's exclusive scan plus the sum of the totals of the tiles before
 // it (each block adds those up itself), and start[ncells].
-
-// A thread's kScanItems consecutive cells from `base` (zero past ncells):
-// 16-byte accesses when the run is whole and aligned, else one word each.
-__device__ __forceinline__ bool whole_run(const uint32_t *p, int base, int ncells) {
-  return base + kScanItems <= ncells && (reinterpret_cast<uintptr_t>(p + base) & 15u) == 0;
-}
-__device__ __forceinline__ void load_run(const uint32_t *__restrict__ count, int base, int ncells,
-                                         uint32_t *v) {
-  if (whole_run(count, base, ncells)) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(count + base);
-#pragma unroll
-    for (int j = 0; j < kScanItems / 4; j++) {
-      const uint4 x = q[j];
-      v[4 * j] = x.x;
-      v[4 * j + 1] = x.y;
-      v[4 * j + 2] = x.z;
-      v[4 * j + 3] = x.w;
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) v[k] = base + k < ncells ? count[base + k] : 0u;
-  }
-}
-__device__ __forceinline__ void store_run(uint32_t *__restrict__ start, int base, int ncells,
-                                          const uint32_t *w) {
-  if (whole_run(start, base, ncells)) {
-    uint4 *q = reinterpret_cast<uint4 *>(start + base);
-#pragma unroll
-    for (int j = 0; j < kScanItems / 4; j++) q[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++)
-      if (base + k < ncells) start[base + k] = w[k];
-  }
-}
 
 static __global__ __launch_bounds__(kScanBlock) void k_tile_totals(const uint32_t *__restrict__ count,
                                                                    uint32_t *__restrict__ bsum,
