@@ -147,7 +147,13 @@ static __global__ void k_rank_gather(const uint32_t *__restrict__ tmp_idx, const
   const uint32_t k = key[src];
   const uint32_t a = start[k], b = start[k + 1];
   uint32_t rank = 0;
-  for (uint32_t t = a; t < b; t++) rank += tmp_id[t] < my ? 1u : 0u;
+  // four reads in flight: cells of dense clusters hold hundreds of agents
+  uint32_t t = a;
+  for (; t + 4 <= b; t += 4) {
+    const int32_t i0 = tmp_id[t], i1 = tmp_id[t + 1], i2 = tmp_id[t + 2], i3 = tmp_id[t + 3];
+    rank += (i0 < my ? 1u : 0u) + (i1 < my ? 1u : 0u) + (i2 < my ? 1u : 0u) + (i3 < my ? 1u : 0u);
+  }
+  for (; t < b; t++) rank += tmp_id[t] < my ? 1u : 0u;
   const uint32_t dst = a + rank;
   id_out[dst] = my;
   if (skey_out) skey_out[dst] = k;
@@ -328,7 +334,14 @@ __device__ __forceinline__ void place_item(int s, const uint32_t *__restrict__ k
     uint32_t rank = 0;
     // stayers: the cell's old range, less the agents that left it
     const uint32_t a = start_old[k], b = start_old[k + 1];
-    for (uint32_t t = a; t < b; t++) rank += (key[t] == k && id_in[t] < my) ? 1u : 0u;
+    uint32_t t = a;
+    for (; t + 4 <= b; t += 4) {  // four reads of each in flight
+      const uint32_t k0 = key[t], k1 = key[t + 1], k2 = key[t + 2], k3 = key[t + 3];
+      const int32_t i0 = id_in[t], i1 = id_in[t + 1], i2 = id_in[t + 2], i3 = id_in[t + 3];
+      rank += ((k0 == k && i0 < my) ? 1u : 0u) + ((k1 == k && i1 < my) ? 1u : 0u) +
+              ((k2 == k && i2 < my) ? 1u : 0u) + ((k3 == k && i3 < my) ? 1u : 0u);
+    }
+    for (; t < b; t++) rank += (key[t] == k && id_in[t] < my) ? 1u : 0u;
     // incoming movers of this generation (each pushed once: the list ends)
     unsigned long long h = ml.head[k];
     while ((uint32_t) (h >> 32) == gen) {
