@@ -237,6 +237,17 @@ class Gen {
   // kept as their float square sums, compared against exact bounds
   // (abl_root_bounds.hpp) instead of taking the root
   std::set<int> sq_vars;  // locals of the function being emitted
+  // The innermost near loop whose test's square sum may stand for
+  // dist(centre.pos, nx.pos) in its body: the loop variable, the centre
+  // agent variable, the position member and the emitted SqLen's name.
+  struct NearSq {
+    int loop_var, centre_var;
+    std::string pos, sq;
+  };
+  std::vector<NearSq> near_sq;
+  // the SqLen of the enclosing near test when `call` is dist() of the loop
+  // agent's and the centre's positions (either order), else ""
+  std::string near_dist_sq(const Expr &call) const;
   static bool root_call(const Expr &e) {
     return e.kind == EK::Call && e.call == CallKind::Builtin &&
            (e.sig == "dist_float2" || e.sig == "dist_float3" || e.sig == "length_float2" ||
@@ -384,6 +395,10 @@ std::string Gen::call(const Expr &e) {
       {"length_float3", "length3"},   {"dist_float2", "dist2"},       {"dist_float3", "dist3"},
       {"normalize_float2", "normalize2"}, {"normalize_float3", "normalize3"}};
   auto g = kGeom.find(n);
+  if (!host && (n == "dist_float2" || n == "dist_float3")) {
+    const std::string q = near_dist_sq(e);
+    if (!q.empty()) return "(F) sqrtf_rn(" + q + ".s)";
+  }
   if (g != kGeom.end()) return host ? n + "(" + args() + ")" : g->second + "<F>(" + args() + ")";
   if (n == "random_float") {
     if (mode == Mode::Device)
@@ -516,7 +531,9 @@ void Gen::stmt(const Stmt &s) {
         std::string a;
         for (size_t k = 0; k < c.a.size(); k++) a += (k ? ", " : "") + expr(*c.a[k]);
         const std::string n = c.sig.substr(0, c.sig.find('_')) + (c.sig.back() == '2' ? "2" : "3");
-        o << nl() << "const SqLen<F> " << ident(s.var.name) << " = sq_" << n << "<F>(" << a << ");";
+        const std::string q = near_dist_sq(c);
+        o << nl() << "const SqLen<F> " << ident(s.var.name) << " = "
+          << (q.empty() ? "sq_" + n + "<F>(" + a + ")" : q) << ";";
         return;
       }
       o << nl() << type_name(t) << " " << ident(s.var.name);
@@ -689,6 +706,24 @@ bool Gen::const_expr(const Expr &e) {
   return constant && !e.type.is_vec();
 }
 
+std::string Gen::near_dist_sq(const Expr &call) const {
+  if (call.kind != EK::Call || call.a.size() != 2 ||
+      (call.sig != "dist_float2" && call.sig != "dist_float3"))
+    return "";
+  auto is_pos = [](const Expr &x, int var, const std::string &pos) {
+    return x.kind == EK::Member && x.name == pos && !x.a.empty() && x.a[0]->kind == EK::Var &&
+           x.a[0]->var.id == var;
+  };
+  const Expr &a = *call.a[0], &b = *call.a[1];
+  for (auto it = near_sq.rbegin(); it != near_sq.rend(); ++it) {
+    // (a - b) and (b - a) square to the same bits, summed in the same order
+    if ((is_pos(a, it->loop_var, it->pos) && is_pos(b, it->centre_var, it->pos)) ||
+        (is_pos(a, it->centre_var, it->pos) && is_pos(b, it->loop_var, it->pos)))
+      return it->sq;
+  }
+  return "";
+}
+
 static bool relational(BinOp op) {
   return op == BinOp::Lt || op == BinOp::Le || op == BinOp::Gt || op == BinOp::Ge;
 }
@@ -754,7 +789,9 @@ std::string Gen::root_compare(const Expr &e) {
   } else {
     std::string a;
     for (size_t k = 0; k < q.a.size(); k++) a += (k ? ", " : "") + expr(*q.a[k]);
-    qs = "sq_" + q.sig.substr(0, q.sig.find('_')) + (q.sig.back() == '2' ? "2" : "3") + "<F>(" + a + ")";
+    qs = near_dist_sq(q);
+    if (qs.empty())
+      qs = "sq_" + q.sig.substr(0, q.sig.find('_')) + (q.sig.back() == '2' ? "2" : "3") + "<F>(" + a + ")";
   }
   const std::string cv = c.type.is_int() ? "(double) (F) (" + expr(c) + ")" : "(double) (" + expr(c) + ")";
   const char *fnm = op == BinOp::Lt ? "len_lt" : op == BinOp::Le ? "len_le" : op == BinOp::Gt ? "len_gt" : "len_ge";
@@ -825,11 +862,29 @@ void Gen::near_loop(const Stmt &s) {
   o << nl();
   for (size_t m = 0; m < nag->members.size(); m++)
     if (need.count(nag->members[m]->name)) o << load_member_nb(nag, (int) m, v, staged);
-  o << nl() << "if (c_.outside(" << v << "." << pos->name << ", " << p << ")) return true;";
+  // the near test on the square sum, kept for dist(nx.pos, centre.pos) in
+  // the body while neither agent variable is assigned there
+  const std::string nsq = fresh();
+  o << nl() << "const SqLen<F> " << nsq << " = sq_dist" << (dim3 ? "3" : "2") << "<F>(" << v << "."
+    << pos->name << ", " << p << ");";
+  o << nl() << "if (c_.outside_sq(" << nsq << ")) return true;";
+  bool reuse = centre_agent.kind == EK::Var;
+  each_stmt(
+      *s.s0,
+      [&](const Stmt &x) {
+        if ((x.kind != SK::Assign && x.kind != SK::AssignOp) || !x.e0) return;
+        const Expr *t = x.e0.get();
+        while ((t->kind == EK::Member || t->kind == EK::Index) && !t->a.empty()) t = t->a[0].get();
+        if (t->kind == EK::Var && (t->var.id == s.var.id || t->var.id == centre_agent.var.id))
+          reuse = false;
+      },
+      [](const Expr &) {});
+  if (reuse) near_sq.push_back({s.var.id, centre_agent.var.id, pos->name, nsq});
   loops.push_back('n');
   if (s.s0->kind == SK::Block) body(s.s0->body);
   else stmt(*s.s0);
   loops.pop_back();
+  if (reuse) near_sq.pop_back();
   o << nl() << "return true;";
   ind--;
   o << nl() << "});";

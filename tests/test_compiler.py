@@ -184,7 +184,9 @@ def test_step_program_shape(tmp_path):
     assert r.returncode == 0, r.stderr
     dev = (out / "steps.hip").read_text()
     assert 'registers program "circle_min"' in dev and '"circle_min",' in dev
-    assert "c_.near(" in dev and "c_.outside(nx.pos, " in dev
+    # the near test, on the square sum of nx.pos - centre
+    m = re.search(r"const SqLen<F> (\w+) = sq_dist2<F>\(nx\.pos, (\w+)\);", dev)
+    assert "c_.near(" in dev and m and f"c_.outside_sq({m.group(1)})" in dev
     assert "static __host__ __device__ double radius_move() { return (double) ((2.0 * r)); }" in dev
     # the near loop walks the stepping agent's own type around it and reads
     # float members only: the step runs on the runtime's LDS tiles
@@ -270,9 +272,13 @@ void main() {
     r = ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out))
     assert r.returncode == 0, r.stderr
     dev = (out / "steps.hip").read_text()
-    assert "const SqLen<F> d = sq_dist2<F>(in.pos, q.pos);" in dev
+    # dist(in.pos, q.pos) in the near loop is the near test's square sum
+    # (same squares, same order), reused rather than summed again
+    m = re.search(r"const SqLen<F> (\w+) = sq_dist2<F>\(q\.pos, (\w+)\);", dev)
+    assert m and f"c_.outside_sq({m.group(1)})" in dev
+    assert f"const SqLen<F> d = {m.group(1)};" in dev
     assert "len_lt<F>(d, (double) (R))" in dev and "len_ge<F>(d, (double) (F) (1))" in dev
-    assert "F e = dist2<F>(in.pos, q.pos);" in dev
+    assert f"F e = (F) sqrtf_rn({m.group(1)}.s);" in dev
     assert "F sp = length2<F>(in.v);" in dev
     assert "len_ge<F>(sq_length2<F>(in.v), (double) ((2.0 * R)))" in dev
 
