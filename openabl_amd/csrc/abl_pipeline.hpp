@@ -244,9 +244,7 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t
 struct MoverLists {
   uint32_t *dirty;                 // per cell: generation of the last build that touched it
   unsigned long long *head;        // per cell: (generation << 32) | first mover, newest first
-  unsigned long long *next;        // per mover: the head it replaced
-  uint32_t *slot;                  // per mover: its slot in the stored order
-  uint32_t *nmov;                  // two counters, used by alternate generations
+  unsigned long long *next;        // per mover (numbered by its slot): the head it replaced
 };
 
 // table[idx] += v for every active lane with `on`, one atomic per distinct
@@ -289,10 +287,9 @@ __device__ __forceinline__ void bin_delta_item(int i, const P *__restrict__ px,
     wave_add(bsum, o / kScanTile, cross, ~0u);
     ml.dirty[k] = gen;
     ml.dirty[o] = gen;
-    // a mover's number is its slot (unique, below n: no shared counter; the
-    // numbering is free, k_place ranks a cell's agents by id)
+    // a mover is numbered by its slot (unique, below n, no shared counter;
+    // the numbering is free, k_place ranks a cell's agents by id)
     const uint32_t m = (uint32_t) i;
-    ml.slot[m] = (uint32_t) i;
     ml.next[m] = atomicExch(&ml.head[k], ((unsigned long long) gen << 32) | m);
   }
 }
@@ -318,12 +315,6 @@ __device__ __forceinline__ void place_item(int s, const uint32_t *__restrict__ k
                                            const MoverLists &ml, uint32_t gen, const GatherCols &g,
                                            const int32_t *__restrict__ id_in,
                                            int32_t *__restrict__ id_out, int n) {
-  // the mover counters start from zero at the next build (binning is done
-  // with this generation's counter)
-  if (s == 0) {
-    ml.nmov[0] = 0u;
-    ml.nmov[1] = 0u;
-  }
   if (s >= n) return;
   const uint32_t k = key[s];
   uint32_t dst;
@@ -331,22 +322,31 @@ __device__ __forceinline__ void place_item(int s, const uint32_t *__restrict__ k
     dst = start_new[k] + ((uint32_t) s - start_old[k]);
   } else {
     const int32_t my = id_in[s];
-    uint32_t rank = 0;
-    // stayers: the cell's old range, less the agents that left it
+    // stayers with a smaller id: the cell's stored range [a, b) is in id
+    // order, so they are the stayers before position p -- the agent's own
+    // slot for a stayer, else the first position holding a larger id
     const uint32_t a = start_old[k], b = start_old[k + 1];
-    uint32_t t = a;
-    for (; t + 4 <= b; t += 4) {  // four reads of each in flight
-      const uint32_t k0 = key[t], k1 = key[t + 1], k2 = key[t + 2], k3 = key[t + 3];
-      const int32_t i0 = id_in[t], i1 = id_in[t + 1], i2 = id_in[t + 2], i3 = id_in[t + 3];
-      rank += ((k0 == k && i0 < my) ? 1u : 0u) + ((k1 == k && i1 < my) ? 1u : 0u) +
-              ((k2 == k && i2 < my) ? 1u : 0u) + ((k3 == k && i3 < my) ? 1u : 0u);
+    uint32_t p = (uint32_t) s;
+    if (k != skey_in[s]) {
+      uint32_t lo = a, hi = b;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (id_in[mid] < my) lo = mid + 1;
+        else hi = mid;
+      }
+      p = lo;
     }
-    for (; t < b; t++) rank += (key[t] == k && id_in[t] < my) ? 1u : 0u;
+    uint32_t rank = 0;
+    uint32_t t = a;
+    for (; t + 4 <= p; t += 4)  // four reads in flight
+      rank += (key[t] == k ? 1u : 0u) + (key[t + 1] == k ? 1u : 0u) + (key[t + 2] == k ? 1u : 0u) +
+              (key[t + 3] == k ? 1u : 0u);
+    for (; t < p; t++) rank += key[t] == k ? 1u : 0u;
     // incoming movers of this generation (each pushed once: the list ends)
     unsigned long long h = ml.head[k];
     while ((uint32_t) (h >> 32) == gen) {
       const uint32_t m = (uint32_t) h;
-      rank += id_in[ml.slot[m]] < my ? 1u : 0u;
+      rank += id_in[m] < my ? 1u : 0u;  // mover m is the agent at slot m
       h = ml.next[m];
     }
     dst = start_new[k] + rank;

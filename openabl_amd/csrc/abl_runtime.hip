@@ -173,11 +173,10 @@ struct AgentStore {
   // build (same grid, same agents) with count/bsum/skey describing it
   uint32_t *skey[2] = {nullptr, nullptr};
   int skey_cur = 0;
-  uint32_t *dirty = nullptr, *mslot = nullptr, *nmov = nullptr;
+  uint32_t *dirty = nullptr;
   unsigned long long *mhead = nullptr, *mnext = nullptr;
   uint32_t gen = 0;
   bool sorted = false;
-  bool nmov_clean = false;
   GridView sgv{};
   size_t sorted_n = 0;
   // neighbour lists (abl_tile.hpp): valid for the stored order while nl_ok
@@ -291,13 +290,13 @@ static void free_store(AgentStore &s) {
     s.id[b] = nullptr;
   }
   void *grid_ptrs[] = {s.key, s.count, s.startb[0], s.startb[1], s.bsum, s.tmp_idx, s.tmp_id,
-                       s.skey[0], s.skey[1], s.dirty, s.mslot, s.nmov, s.mhead, s.mnext};
+                       s.skey[0], s.skey[1], s.dirty, s.mhead, s.mnext};
   for (void *p : grid_ptrs) {
     if (p) (void) hipFree(p);
   }
   s.key = s.count = s.startb[0] = s.startb[1] = s.bsum = s.tmp_idx = nullptr;
   s.tmp_id = nullptr;
-  s.skey[0] = s.skey[1] = s.dirty = s.mslot = s.nmov = nullptr;
+  s.skey[0] = s.skey[1] = s.dirty = nullptr;
   s.mhead = s.mnext = nullptr;
   s.sorted = false;
   void *nl_ptrs[] = {s.nl_idx, s.nl_len, s.nl_pos0, s.nl_gate, s.nl_perm, s.nl_prow0};
@@ -338,7 +337,7 @@ constexpr size_t kListMetaBytes = 2 + 4 + 16 + 24;
 static size_t store_bytes(const AgentStore &s) {
   size_t b = 2 * s.cap * col_bytes(s);
   b += 2 * s.cap * 4;                      // ids
-  b += s.cap * (4 + 4 + 4 + 8 + 4 + 8);    // key, tmp_idx, tmp_id, skey x2, mover slot, next
+  b += s.cap * (4 + 4 + 4 + 8 + 8);        // key, tmp_idx, tmp_id, skey x2, mover next
   const size_t tiles = (s.cells_cap + 2047) / 2048;
   b += s.cells_cap * (4 + 8 + 4 + 8) + std::max<size_t>(tiles, 1) * 4;  // count, start x2, dirty, head; bsum
   if (s.nl_idx) b += s.nl_ncap * ((size_t) s.nl_cap * 2 + kListMetaBytes) + 16 * 4;
@@ -373,7 +372,7 @@ static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
     int rc = grow((void **) &s.id[b], 4);
     if (rc) return rc;
   }
-  void *ptrs[] = {s.key, s.tmp_idx, s.tmp_id, s.skey[0], s.skey[1], s.mslot, s.mnext};
+  void *ptrs[] = {s.key, s.tmp_idx, s.tmp_id, s.skey[0], s.skey[1], s.mnext};
   for (void *p : ptrs) {
     if (p) HIP_TRY(hipFree(p));
   }
@@ -382,12 +381,7 @@ static int reserve_store(abl_sim *sim, AgentStore &s, size_t n) {
   HIP_TRY(hipMalloc(&s.tmp_id, cap * 4));
   HIP_TRY(hipMalloc(&s.skey[0], cap * 4));
   HIP_TRY(hipMalloc(&s.skey[1], cap * 4));
-  HIP_TRY(hipMalloc(&s.mslot, cap * 4));
   HIP_TRY(hipMalloc(&s.mnext, cap * 8));
-  if (!s.nmov) {
-    HIP_TRY(hipMalloc(&s.nmov, 2 * 4));
-    HIP_TRY(hipMemsetAsync(s.nmov, 0, 2 * 4, sim->stream));
-  }
   s.cap = cap;
   s.sorted = false;
   s.nl_ok = false;
@@ -669,7 +663,7 @@ static void launch_bin(abl_sim *sim, AgentStore &s, bool delta, int n,
   const P *py = (const P *) col_cur(s, pc + 1);
   const P *pz = (const P *) (s.dim == 3 ? col_cur(s, pc + 2) : nullptr);
   if (delta) {
-    MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
+    MoverLists ml{s.dirty, s.mhead, s.mnext};
     hipLaunchKernelGGL(k_bin_delta<P>, grid_blocks(n, 256), 256, 0, sim->stream, px, py, pz, s.gv,
                        s.skey[s.skey_cur], s.key, s.count, s.bsum, ml, s.gen, n, gate);
   } else {
@@ -693,7 +687,7 @@ static void launch_delta(abl_sim *sim, AgentStore &s, const uint32_t *gate, uint
   hipLaunchKernelGGL(k_scan_apply, std::max(ntiles, 1), kScanBlock, 0, st, s.count, s.bsum,
                      start_new, nc, gate);
   if (n > 0) {
-    MoverLists ml{s.dirty, s.mhead, s.mnext, s.mslot, s.nmov};
+    MoverLists ml{s.dirty, s.mhead, s.mnext};
     hipLaunchKernelGGL(k_place, grid_blocks(n, 256), 256, 0, st, s.key, s.skey[s.skey_cur],
                        s.skey[1 - s.skey_cur], start_old, start_new, ml, s.gen, gc,
                        s.id[s.id_cur], s.id[1 - s.id_cur], n, gate);
@@ -713,7 +707,7 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
   const int n = (int) s.n;
   const int nc = (int) s.ncells;
   hipStream_t st = sim->stream;
-  const bool delta = sim->grid_delta && n > 0 && s.nmov && s.sorted && s.sorted_n == s.n &&
+  const bool delta = sim->grid_delta && n > 0 && s.mnext && s.sorted && s.sorted_n == s.n &&
                      same_grid(s.gv, s.sgv);
   s.gen += 1;
   if (s.gen == 0) s.gen = 1;  // tag 0 marks never-touched cells and lists
@@ -729,9 +723,7 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
     gc.elem[c] = s.cols[c].elem;
   }
   if (delta) {
-    if (!s.nmov_clean) HIP_TRY(hipMemsetAsync(s.nmov, 0, 2 * 4, st));
     launch_delta(sim, s, nullptr, start_old, start_new, gc);
-    s.nmov_clean = true;
   } else {
     // the counts of the previous build are still in `count`
     HIP_TRY(hipMemsetAsync(s.count, 0, (size_t) (nc + 1) * 4, st));
@@ -749,7 +741,6 @@ static int build_grid(abl_sim *sim, AgentStore &s) {
                          s.key, start_new, gc, s.id[1 - s.id_cur], n, s.skey[1 - s.skey_cur],
                          s.count);
     }
-    s.nmov_clean = false;
   }
   HIP_TRY(hipGetLastError());
   if (n > 0) {
@@ -1396,7 +1387,7 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
   // program's launch function can run it; else four gated launches
   DeltaArgs d{};
   int launched = 0;
-  if (sim->fused_rebuild && b.n > 0 && b.nmov) {
+  if (sim->fused_rebuild && b.n > 0 && b.mnext) {
     const int pc = b.pos_col;
     d.px = col_cur(b, pc);
     d.py = col_cur(b, pc + 1);
@@ -1407,7 +1398,7 @@ static int gated_rebuild(abl_sim *sim, AgentStore &b, const StepDesc &st, const 
     d.key = b.key;
     d.count = b.count;
     d.bsum = b.bsum;
-    d.ml = MoverLists{b.dirty, b.mhead, b.mnext, b.mslot, b.nmov};
+    d.ml = MoverLists{b.dirty, b.mhead, b.mnext};
     d.gen = b.gen;
     d.n = (int) b.n;
     d.start_old = start_cur;
