@@ -487,7 +487,7 @@ def run_serving(args):
         t += rng.exponential(args.serve_interval)
         reqs.append((t, prog, prec, p, steps))
     srv = Server(hbm_budget=int(args.serve_budget_gb * (1 << 30)), max_active=args.serve_active,
-                 chunk_steps=25)
+                 chunk_steps=args.serve_chunk)
     t0 = time.perf_counter()
     ids = []
     for (at, prog, prec, p, steps) in reqs:
@@ -509,7 +509,7 @@ def run_serving(args):
                        "steps": "uniform 100-1000",
                        "mean_interarrival_s": args.serve_interval,
                        "budget_gb": args.serve_budget_gb,
-                       "max_active": args.serve_active}}
+                       "max_active": args.serve_active, "chunk_steps": args.serve_chunk}}
 
 
 def main():
@@ -543,6 +543,8 @@ def main():
     ap.add_argument("--serve-interval", type=float, default=0.05)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-active", type=int, default=8)
+    ap.add_argument("--serve-chunk", type=int, default=25,
+                    help="timesteps a served run is given at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -134,7 +134,10 @@ int abl_hip_get_param(abl_sim *sim, const char *name, double *value);
  * order), "hip.tile" (LDS tile shape, 0 = untiled), "hip.lds_slack" (LDS
  * records per tile as a multiple of the expected halo), "hip.grid" ("delta"
  * or "full" re-sort), "hip.nlist" (1: neighbour lists with a skin for tiled
- * near loops over the agent's own type, 0: walk the stencil every step),
+ * near loops over the agent's own type, 0: walk the stencil every step;
+ * left unset, lists are used and given up for a type whose lists were
+ * rebuilt on more than half of the 16 timesteps after their first build --
+ * decided at that fixed timestep, so results do not depend on timing),
  * "hip.skin" (list skin as a fraction of the radius), "hip.list_cap" (list
  * entries per agent; a longer list falls back to the stencil walk),
  * "hip.cell_inflate" (cell widening factor; list mode widens by 1 + skin),

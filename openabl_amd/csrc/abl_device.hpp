@@ -116,7 +116,7 @@ struct NListArgs {
   uint4 *prow0;         // [n]
   double *pos0;         // [dim][n] positions at the build
   uint32_t *gate;       // [2] rebuild flags by timestep parity
-  uint32_t *stats;      // [0]: longest list at the last build
+  uint32_t *stats;      // [0]: longest list at the last build; [2]: gated rebuilds run
   int n, cap, parity;
   int stride;           // agents per row block of idx (>= n)
   float rc_list;        // (radius + skin) in cell units, widened

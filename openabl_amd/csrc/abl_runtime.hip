@@ -186,9 +186,19 @@ struct AgentStore {
   uint4 *nl_prow0 = nullptr;    // per own slot: that agent's first list row
   double *nl_pos0 = nullptr;
   uint32_t *nl_gate = nullptr;  // [0..1] rebuild gates by timestep parity, [2] longest list,
+                                // [4] gated rebuilds run,
                                 // [8..15] work queue of the one-launch rebuild
   int nl_cap = 0;               // entries per agent
   size_t nl_ncap = 0;           // agents allocated
+  // the rebuild rate of list mode (list_rate): a pinned copy of the device's
+  // count of gated rebuilds, the event after that copy, the previous sample
+  uint32_t *nl_host = nullptr;
+  hipEvent_t nl_ev = nullptr;
+  bool nl_pending = false;
+  bool nl_off = false;          // lists given up for this type (rebuilt nearly every step)
+  bool nl_kept = false;         // the rebuild rate was measured and the lists kept
+  uint64_t nl_t_req = 0, nl_t_prev = 0;
+  uint32_t nl_c_prev = 0;
   // run-time add/remove: ids stop being 0..n-1 once agents die or are born
   int64_t next_id = 0;
   bool ids_dense = true;
@@ -212,6 +222,7 @@ struct abl_sim {
   bool grid_delta = true;  // "hip.grid": delta re-sort when the stored order allows it
   float lds_slack = 2.0f;  // "hip.lds_slack": LDS tile records per expected halo agent
   bool nlist = true;       // "hip.nlist": neighbour lists for steps that support them
+  bool nlist_set = false;  // "hip.nlist" given: lists kept whatever their rebuild rate
   double skin = kDefaultSkin;  // "hip.skin": list skin as a fraction of the near radius
   int list_cap = 0;        // "hip.list_cap": entries per list (0: sized from the data)
   double cell_inflate = 0.0;  // "hip.cell_inflate": cell widening factor (0: automatic)
@@ -311,6 +322,11 @@ static void free_store(AgentStore &s) {
   s.nl_cap = 0;
   s.nl_ncap = 0;
   s.nl_ok = false;
+  if (s.nl_ev) (void) hipEventDestroy(s.nl_ev);
+  if (s.nl_host) (void) hipHostFree(s.nl_host);
+  s.nl_ev = nullptr;
+  s.nl_host = nullptr;
+  s.nl_pending = false;
   for (int c = 0; c < kMaxCols; c++) {
     if (s.bmem[c]) (void) hipFree(s.bmem[c]);
     s.bmem[c] = nullptr;
@@ -920,6 +936,7 @@ extern "C" int abl_hip_set_config(abl_sim *sim, const char *key, const char *val
     if (v == "1" || v == "true") sim->nlist = true;
     else if (v == "0" || v == "false") sim->nlist = false;
     else return fail(ABL_ERR_ARG, "hip.nlist must be 0 or 1");
+    sim->nlist_set = true;
   } else if (k == "hip.cell_inflate") {
     const double f = std::atof(value);
     if (!(f == 0.0 || (f >= 1.0 && f <= 2.0))) return fail(ABL_ERR_ARG, "hip.cell_inflate must be 0 or 1..2");
@@ -1284,7 +1301,53 @@ static int apply_births(abl_sim *sim, AgentStore &parent, AgentStore &child, int
 
 static bool list_eligible(const abl_sim *sim, const StepDesc &st, const AgentStore &a) {
   return sim->nlist && st.nlist && st.near_agent == st.agent && sim->order == 0 && sim->tile != 0 &&
-         !st.removes && !st.adds && a.n > 0 && a.pos_col >= 0 && a.n < (size_t) INT32_MAX;
+         !st.removes && !st.adds && a.n > 0 && a.pos_col >= 0 && a.n < (size_t) INT32_MAX &&
+         !a.nl_off;
+}
+
+// With "hip.nlist" left at its default, lists are given up for an agent type
+// whose gated rebuild runs on most timesteps (its agents outrun the skin
+// every step): a re-sort and list build per timestep then costs more than
+// the stencil walk, which gives the same results.  The device counts the
+// rebuilds it runs (NListArgs::stats[2]; the list entry reads the count
+// while it waits anyway).  kListSample timesteps later the host copies the
+// count to pinned memory behind an event, and it reads the copy when it has
+// issued another kListSample timesteps (waiting for the event if need be:
+// the device still has those timesteps queued, so the wait costs it
+// nothing).  The decision is thus taken at a fixed timestep, whatever the
+// host's and the device's pace, and runs give the same bits alone, batched
+// or served.  It is taken once per type: a type that stops outrunning its
+// skin later keeps the stencil walk.
+constexpr uint64_t kListSample = 16;
+static int list_rate(abl_sim *sim, AgentStore &b) {
+  if (sim->nlist_set || b.nl_off || b.nl_kept) return ABL_OK;
+  if (!b.nl_host) {
+    HIP_TRY(hipHostMalloc((void **) &b.nl_host, 4, hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&b.nl_ev, hipEventDisableTiming));
+  }
+  if (b.nl_pending) {
+    if (sim->timestep - b.nl_t_req < kListSample) return ABL_OK;
+    HIP_TRY(hipEventSynchronize(b.nl_ev));
+    const uint32_t c = *b.nl_host;
+    const uint64_t dt = b.nl_t_req - b.nl_t_prev;
+    if (dt >= kListSample && 2 * (uint64_t) (c - b.nl_c_prev) > dt) {
+      b.nl_off = true;
+      b.sorted = false;  // the stencil walk starts from a full sort
+    } else {
+      b.nl_kept = true;
+    }
+    b.nl_c_prev = c;
+    b.nl_t_prev = b.nl_t_req;
+    b.nl_pending = false;
+    return ABL_OK;
+  }
+  if (sim->timestep - b.nl_t_prev >= kListSample) {
+    HIP_TRY(hipMemcpyAsync(b.nl_host, b.nl_gate + 4, 4, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(hipEventRecord(b.nl_ev, sim->stream));
+    b.nl_t_req = sim->timestep;
+    b.nl_pending = true;
+  }
+  return ABL_OK;
 }
 
 static int reserve_lists(abl_sim *sim, AgentStore &s, int cap) {
@@ -1353,9 +1416,14 @@ static int enter_lists(abl_sim *sim, AgentStore &b, const StepDesc &st, const St
     L.nl = &nl;
     st.launch(L);
     HIP_TRY(hipGetLastError());
-    uint32_t longest = 0;
-    HIP_TRY(hipMemcpyAsync(&longest, b.nl_gate + 2, 4, hipMemcpyDeviceToHost, sim->stream));
+    uint32_t words[3] = {0, 0, 0};  // longest list; -; gated rebuilds run so far
+    HIP_TRY(hipMemcpyAsync(words, b.nl_gate + 2, sizeof(words), hipMemcpyDeviceToHost, sim->stream));
     HIP_TRY(hipStreamSynchronize(sim->stream));
+    const uint32_t longest = words[0];
+    // the rebuild rate is measured from here (list_rate)
+    b.nl_c_prev = words[2];
+    b.nl_t_prev = sim->timestep;
+    b.nl_pending = false;
     // grow to the longest list, but to at most four times the estimate: an
     // agent in a dense cluster with a longer list walks the stencil instead
     const int grow = std::min((int) (1.25 * longest) + 16, 4 * want);
@@ -1536,6 +1604,10 @@ static int launch_step(abl_sim *sim, const StepDesc &st, uint32_t ordinal) {
       int rc = gated ? gated_rebuild(sim, b, st, L, r, rl) : enter_lists(sim, b, st, L, r, rl);
       span_close(sim, sp);
       if (rc) return rc;
+      if (gated) {
+        rc = list_rate(sim, b);
+        if (rc) return rc;
+      }
       nl_step = list_args(sim, b, r, rl);
       L.nl_mode = 2;
       L.nl = &nl_step;

@@ -994,6 +994,7 @@ template <class M, int TX, int TY, int TZ, int RM>
 __global__ __launch_bounds__(256) void tiled_list_build(typename M::Args a, NListArgs nl) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (nl.gated && nl.gate[nl.parity] == 0u) return;
+  if (nl.gated && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.stats[2], 1u);  // rebuilds run
   list_copy_starts(nl);
   const uint32_t longest =
       list_build_tile<M, TX, TY, TZ, RM>(a, nl, xcd_block(blockIdx.x, gridDim.x), smem);
@@ -1016,6 +1017,7 @@ __global__ __launch_bounds__(256) void tiled_rebuild(typename M::Args a, NListAr
   extern __shared__ __align__(16) unsigned char smem[];
   if (nl.gate[nl.parity] == 0u) return;
   if (__hip_atomic_load(&d.q->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&nl.stats[2], 1u);  // rebuilds run
   __shared__ uint32_t lds[kWave + 1];
   const unsigned nt = tile_count<TX, TY, TZ, M::DIM>(a.g);
   const uint32_t chunks = (uint32_t) (d.n + 255) / 256u;

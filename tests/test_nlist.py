@@ -214,3 +214,22 @@ def test_two_live_simulations_equal_alone():
     b.close()
     for x, y in zip(got, alone):
         assert field_bytes(x) == field_bytes(y)
+
+
+def test_lists_given_up_when_rebuilt_every_step():
+    """With hip.nlist left at its default, an agent type whose lists are
+    rebuilt on most timesteps (circle's agents outrun the skin every step)
+    goes back to the stencil walk after a sample of the device's rebuild
+    count; the runs before, across and after the switch equal the stencil
+    walk bit for bit.  With hip.nlist=1 given, the lists stay (same bytes)."""
+    print("case nlist given up: circle 65536, 120 steps", flush=True)
+    params = circle_params(num_agents=65536, rho=0.1024, r=2.5)
+    steps = (20, 60, 120)
+    cells = {"hip.cell_inflate": 1.0 + 0.05}  # the list mode's cells in both runs
+    a = run("circle", "f64", params, steps, **cells)
+    b = run("circle", "f64", params, steps, **dict(cells, **{"hip.nlist": 0}))
+    for t, x, y in zip(steps, a, b):
+        assert field_bytes(x) == field_bytes(y), t
+    c = run("circle", "f64", params, (40,), **dict(cells, **{"hip.nlist": 1}))
+    assert field_bytes(c[0]) == field_bytes(run("circle", "f64", params, (40,),
+                                                **dict(cells, **{"hip.nlist": 0}))[0])
