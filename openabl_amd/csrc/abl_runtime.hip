@@ -518,15 +518,25 @@ __global__ void k_birth_list(const uint8_t *__restrict__ born, const uint32_t *_
 }
 
 // Append the newborns after the child type's agents in ascending parent id;
-// they take ids next_id + rank.
-__global__ void k_birth_append(const int32_t *__restrict__ pid, const uint32_t *__restrict__ slot,
-                               int nb, GatherCols g, int32_t *__restrict__ child_id, int n_child,
-                               int32_t next_id) {
+// they take ids next_id + rank.  A newborn's rank counts the smaller parent
+// ids among all newborns, read 256 at a time through LDS (blockDim 256).
+__global__ __launch_bounds__(256) void k_birth_append(const int32_t *__restrict__ pid,
+                                                      const uint32_t *__restrict__ slot, int nb,
+                                                      GatherCols g, int32_t *__restrict__ child_id,
+                                                      int n_child, int32_t next_id) {
+  __shared__ int32_t chunk[256];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nb) return;
-  const int32_t my = pid[k];
+  const int32_t my = k < nb ? pid[k] : 0;
   uint32_t rank = 0;
-  for (int q = 0; q < nb; q++) rank += pid[q] < my ? 1u : 0u;
+  for (int base = 0; base < nb; base += 256) {  // every thread of the block takes part
+    const int q = base + (int) threadIdx.x;
+    if (q < nb) chunk[threadIdx.x] = pid[q];
+    __syncthreads();
+    const int m = nb - base < 256 ? nb - base : 256;
+    for (int j = 0; j < m; j++) rank += chunk[j] < my ? 1u : 0u;
+    __syncthreads();
+  }
+  if (k >= nb) return;
   const uint32_t dst = (uint32_t) n_child + rank;
   copy_cols(g, slot[k], dst);
   child_id[dst] = next_id + (int32_t) rank;
