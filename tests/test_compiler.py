@@ -293,3 +293,35 @@ def test_generated_main_links_step_library():
     assert "-Wl,--no-as-needed -lsteps" in open(os.path.join(gen, "build.sh")).read()
     r = subprocess.run(["readelf", "-d", os.path.join(gen, "main")], capture_output=True, text=True)
     assert "[libsteps.so]" in r.stdout
+
+
+def test_near_sum_reused_only_for_loop_and_centre_positions(tmp_path):
+    """dist() of the loop agent's and the centre's positions, in either
+    order, is the near test's square sum; dist() of any other pair (another
+    member, a shifted position) is summed as written."""
+    src = """
+agent P { position float2 pos; float2 v; }
+environment { max: float2(10) }
+step s(P in -> out) {
+  float a = 0;
+  for (P q : near(in, 3)) {
+    a += dist(q.pos, in.pos);
+    a += dist(in.v, q.pos);
+    a += dist(in.pos + float2(1, 0), q.pos);
+  }
+  out.v = float2(a, 0);
+}
+void main() {
+  for (int i : 0..10) { add(P { pos: float2(i, 0), v: float2(1, 0) }); }
+  simulate(3) { s }
+}
+"""
+    out = tmp_path / "n"
+    r = ablc("-i", model(tmp_path, src), "-b", "hip", "-o", str(out))
+    assert r.returncode == 0, r.stderr
+    dev = (out / "steps.hip").read_text()
+    m = re.search(r"const SqLen<F> (\w+) = sq_dist2<F>\(q\.pos, (\w+)\);", dev)
+    assert m and f"c_.outside_sq({m.group(1)})" in dev
+    assert f"(F) sqrtf_rn({m.group(1)}.s)" in dev
+    assert "dist2<F>(in.v, q.pos)" in dev
+    assert "dist2<F>(v2_add<F>(in.pos, " in dev
